@@ -1,0 +1,170 @@
+"""ctypes binding of the gfx950 C-ABI library ``libpf_hip.so``.
+
+The library has a plain C interface (``include/pf_hip.h``): device pointers,
+sizes and a ``hipStream_t``.  This module loads it, mirrors its structs and
+maps status codes onto the reference's exception types
+(ValueError for bad inputs, as ``powerfoam/warp_interop.py:14-15`` and
+``powerfoam/rasterize.py:1620-1637`` raise; RuntimeError for device errors).
+There is no CPU fallback: if the library is missing the import fails.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpf_hip.so")
+
+PF_OK, PF_ERR_ARG, PF_ERR_DTYPE, PF_ERR_HIP, PF_ERR_OVERFLOW = 0, 1, 2, 3, 4
+PF_STATUS_STEP_CAP, PF_STATUS_LIST_CAP = 1, 2
+PF_F32, PF_F16 = 0, 1
+
+c_void_p = ctypes.c_void_p
+c_int32 = ctypes.c_int32
+c_int64 = ctypes.c_int64
+c_float = ctypes.c_float
+
+
+class PfCamera(ctypes.Structure):
+    _fields_ = [
+        ("eye", c_float * 3),
+        ("right", c_float * 3),
+        ("up", c_float * 3),
+        ("start_cell", c_int32),
+        ("width", c_int32),
+        ("height", c_int32),
+    ]
+
+
+class PfScene(ctypes.Structure):
+    _fields_ = [
+        ("spheres", c_void_p),
+        ("nsigmas", c_void_p),
+        ("texel_sites", c_void_p),
+        ("texel_rgb", c_void_p),
+        ("texel_height", c_void_p),
+        ("adjacency", c_void_p),
+        ("adjacency_offsets", c_void_p),
+        ("faces", c_void_p),
+        ("num_cells", c_int64),
+        ("num_edges", c_int64),
+        ("num_texel_sites", c_int32),
+        ("attr_dtype", c_int32),
+        ("face_dtype", c_int32),
+        ("reserved", c_int32),
+        ("texel_rgb_cam_stride", c_int64),
+    ]
+
+
+class PfWalkOut(ctypes.Structure):
+    _fields_ = [
+        ("color", c_void_p),
+        ("log_t", c_void_p),
+        ("depth", c_void_p),
+        ("visits", c_void_p),
+        ("seq_hash", c_void_p),
+        ("seq", c_void_p),
+        ("seq_cap", c_int32),
+        ("reserved", c_int32),
+        ("status", c_void_p),
+    ]
+
+
+class PfGrads(ctypes.Structure):
+    _fields_ = [
+        ("spheres", c_void_p),
+        ("nsigmas", c_void_p),
+        ("texel_sites", c_void_p),
+        ("texel_rgb", c_void_p),
+        ("texel_height", c_void_p),
+        ("texel_rgb_cam_stride", c_int64),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/pf_hip.h
+_SIGS = {
+    "pf_device_count": (c_int32, [ctypes.POINTER(c_int32)]),
+    "pf_status_string": (ctypes.c_char_p, [c_int32]),
+    "pf_walk_forward": (
+        c_int32,
+        [ctypes.POINTER(PfScene), c_void_p, c_int32, c_int32, c_int32, c_float, c_float, c_int32,
+         ctypes.POINTER(PfWalkOut), c_void_p],
+    ),
+    "pf_walk_backward": (
+        c_int32,
+        [ctypes.POINTER(PfScene), c_void_p, c_int32, c_int32, c_int32, c_float, c_int32, c_void_p, c_void_p,
+         c_void_p, c_void_p, ctypes.POINTER(PfGrads), c_void_p, c_void_p],
+    ),
+    "pf_walk_faces": (
+        c_int32,
+        [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p, c_int32, c_void_p],
+    ),
+    "pf_start_cells": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_int32, c_void_p, c_void_p]),
+    "pf_sh_forward": (
+        c_int32,
+        [c_void_p, c_int32, c_void_p, c_int64, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_int64, c_void_p],
+    ),
+    "pf_sh_backward": (
+        c_int32,
+        [c_void_p, c_int32, c_void_p, c_int64, c_int32, c_void_p, c_int32, c_void_p, c_int64, c_void_p, c_void_p],
+    ),
+}
+
+_lib = None
+
+
+def load():
+    """Load libpf_hip.so (once).  Raises ImportError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run dynamic-powerfoam_amd/build_lib.py (hipcc, gfx950)"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def check(rc, what):
+    if rc == PF_OK:
+        return
+    msg = load().pf_status_string(rc).decode()
+    if rc in (PF_ERR_ARG, PF_ERR_DTYPE, PF_ERR_OVERFLOW):
+        raise ValueError(f"{what}: {msg}")
+    raise RuntimeError(f"{what}: {msg}")
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL).  CPU tensors are rejected
+    as the reference rejects them (warp_interop.py:14-15)."""
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor):
+        raise ValueError("expected a torch.Tensor")
+    if t.device.type != "cuda":
+        raise ValueError("Only CUDA tensors are supported as inputs")
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return t.data_ptr()
+
+
+def stream_ptr(device=None):
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def dtype_code(dt):
+    if dt == torch.float32:
+        return PF_F32
+    if dt == torch.float16:
+        return PF_F16
+    raise ValueError(f"Unsupported attribute dtype: {dt}")

@@ -1,0 +1,271 @@
+"""Power-cell walk — the drop-in for ``powerfoam/raytrace.py``'s ``RayTracer``.
+
+``RayTracer.benchmark`` keeps the reference's signature and return value
+(``powerfoam/raytrace.py:247-309``) and runs the hand-written gfx950 walk
+kernel (``csrc/walk.hip``) through the C ABI on torch's current stream.
+Extensions, not changes: ``forward`` renders a batch of cameras, returns
+colour, alpha and depth (plus optional visited-cell sequences) and is
+differentiable through ``WalkGradFn``, whose backward replays each ray's walk
+from a compact per-ray checkpoint.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from .camera import cameras_to_device
+
+TILE_WIDTH = 8
+TILE_SIZE = TILE_WIDTH * TILE_WIDTH
+DEFAULT_MAX_STEPS = 1 << 16
+
+
+def _contig(t, dtype=None):
+    if dtype is not None and t.dtype != dtype:
+        t = t.to(dtype)
+    return t.contiguous()
+
+
+class WalkScene:
+    """Device arrays of one scene in the kernels' layout, plus the C struct.
+
+    spheres = [points, radii] (f32) and nsigmas = [normals, density]
+    (attribute dtype), as packed at raytrace.py:274-280.
+    """
+
+    def __init__(self, points, radii, density, normals, texel_sites, texel_rgb, texel_height, adjacency,
+                 adjacency_offsets, faces, attr_dtype=torch.float32):
+        dev = points.device
+        if dev.type != "cuda":
+            raise ValueError("Only CUDA tensors are supported as inputs")
+        n = points.shape[0]
+        self.num_cells = n
+        self.spheres = _contig(torch.cat([points.detach(), radii.detach()[:, None]], dim=-1), torch.float32)
+        self.nsigmas = _contig(torch.cat([normals.detach(), density.detach()[:, None]], dim=-1), attr_dtype)
+        self.texel_sites = _contig(texel_sites.detach(), attr_dtype)
+        self.texel_rgb = _contig(texel_rgb.detach(), attr_dtype)
+        self.texel_height = _contig(texel_height.detach(), attr_dtype)
+        if self.texel_sites.dim() != 3 or self.texel_sites.shape[0] != n:
+            raise ValueError("texel_sites must be (N, K, 3)")
+        self.K = self.texel_sites.shape[1]
+        if adjacency.dtype not in (torch.int32, torch.int64) or adjacency_offsets.dtype not in (torch.int32,
+                                                                                                torch.int64):
+            raise ValueError("adjacency / adjacency_offsets must be integer tensors")
+        self.adjacency = _contig(adjacency, torch.int32)
+        self.offsets = _contig(adjacency_offsets, torch.int32)
+        if self.offsets.numel() != n + 1:
+            raise ValueError("adjacency_offsets must have N+1 entries")
+        self.faces = _contig(faces)
+        if self.faces.dtype not in (torch.float16, torch.float32) or self.faces.shape != (self.adjacency.numel(), 4):
+            raise ValueError("adjacency_diff must be (E, 4) float16 or float32")
+        s = _lib.PfScene()
+        s.spheres = _lib.ptr(self.spheres)
+        s.nsigmas = _lib.ptr(self.nsigmas)
+        s.texel_sites = _lib.ptr(self.texel_sites)
+        s.texel_rgb = _lib.ptr(self.texel_rgb)
+        s.texel_height = _lib.ptr(self.texel_height)
+        s.adjacency = _lib.ptr(self.adjacency)
+        s.adjacency_offsets = _lib.ptr(self.offsets)
+        s.faces = _lib.ptr(self.faces)
+        s.num_cells = n
+        s.num_edges = self.adjacency.numel()
+        s.num_texel_sites = self.K
+        s.attr_dtype = _lib.dtype_code(attr_dtype)
+        s.face_dtype = _lib.dtype_code(self.faces.dtype)
+        self.c = s
+
+
+def start_cells_device(points, radii, cams_dev):
+    """Write each camera's start cell (benchmark.py:320-324) into the device
+    pf_camera array on the GPU, without a host sync."""
+    lib = _lib.load()
+    B = cams_dev.shape[0]
+    ws = torch.empty(B, dtype=torch.int64, device=cams_dev.device)
+    pts = _contig(points.detach(), torch.float32)
+    rad = _contig(radii.detach(), torch.float32)
+    rc = lib.pf_start_cells(_lib.ptr(pts), _lib.ptr(rad), pts.shape[0], _lib.ptr(cams_dev), B, _lib.ptr(ws),
+                            _lib.stream_ptr())
+    _lib.check(rc, "pf_start_cells")
+    return cams_dev
+
+
+def walk_faces(points, radii, adjacency, adjacency_offsets, dtype=torch.float16):
+    """Walk faces (benchmark.py:295-302) computed on the GPU: (E, 4)
+    [p_j - p_i, pm_j - pm_i], pm = 0.5(|p|^2 - r^2)."""
+    lib = _lib.load()
+    pts = _contig(points.detach(), torch.float32)
+    rad = _contig(radii.detach(), torch.float32)
+    adj = _contig(adjacency, torch.int32)
+    off = _contig(adjacency_offsets, torch.int32)
+    faces = torch.empty(adj.numel(), 4, dtype=dtype, device=pts.device)
+    rc = lib.pf_walk_faces(_lib.ptr(pts), _lib.ptr(rad), pts.shape[0], _lib.ptr(adj), _lib.ptr(off), adj.numel(),
+                           _lib.ptr(faces), _lib.dtype_code(dtype), _lib.stream_ptr())
+    _lib.check(rc, "pf_walk_faces")
+    return faces
+
+
+def launch_walk_forward(scene, cams_dev, width, height, transmittance_threshold=1e-3, depth_quantile=0.5,
+                        max_steps=DEFAULT_MAX_STEPS, want_log_t=True, want_depth=True, want_sequences=False,
+                        seq_cap=0):
+    lib = _lib.load()
+    B = cams_dev.shape[0]
+    dev = cams_dev.device
+    color = torch.empty(B, height, width, 3, dtype=torch.float32, device=dev)
+    log_t = torch.empty(B, height, width, dtype=torch.float32, device=dev) if want_log_t else None
+    depth = torch.empty(B, height, width, dtype=torch.float32, device=dev) if want_depth else None
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    visits = seq_hash = seq = None
+    if want_sequences:
+        visits = torch.empty(B, height, width, dtype=torch.int32, device=dev)
+        seq_hash = torch.empty(B, height, width, dtype=torch.int64, device=dev)
+        if seq_cap > 0:
+            seq = torch.full((B, height, width, seq_cap), -1, dtype=torch.int32, device=dev)
+    out = _lib.PfWalkOut()
+    out.color = _lib.ptr(color)
+    out.log_t = _lib.ptr(log_t)
+    out.depth = _lib.ptr(depth)
+    out.visits = _lib.ptr(visits)
+    out.seq_hash = _lib.ptr(seq_hash)
+    out.seq = _lib.ptr(seq)
+    out.seq_cap = seq_cap if seq is not None else 0
+    out.status = _lib.ptr(status)
+    rc = lib.pf_walk_forward(ctypes.byref(scene.c), _lib.ptr(cams_dev), B, width, height,
+                             float(transmittance_threshold), float(depth_quantile), int(max_steps),
+                             ctypes.byref(out), _lib.stream_ptr())
+    _lib.check(rc, "pf_walk_forward")
+    return dict(color=color, log_t=log_t, depth=depth, visits=visits, seq_hash=seq_hash, seq=seq, status=status)
+
+
+def launch_walk_backward(scene, cams_dev, width, height, color, log_t, grad_color, grad_log_t,
+                         transmittance_threshold=1e-3, max_steps=DEFAULT_MAX_STEPS, grads=None):
+    lib = _lib.load()
+    n, K = scene.num_cells, scene.K
+    dev = cams_dev.device
+    if grads is None:
+        grads = dict(
+            spheres=torch.zeros(n, 4, dtype=torch.float32, device=dev),
+            nsigmas=torch.zeros(n, 4, dtype=torch.float32, device=dev),
+            texel_sites=torch.zeros(n, K, 3, dtype=torch.float32, device=dev),
+            texel_rgb=torch.zeros(n, K, 3, dtype=torch.float32, device=dev),
+            texel_height=torch.zeros(n, K, dtype=torch.float32, device=dev),
+        )
+    g = _lib.PfGrads()
+    for k in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
+        setattr(g, k, _lib.ptr(grads[k]))
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    rc = lib.pf_walk_backward(ctypes.byref(scene.c), _lib.ptr(cams_dev), cams_dev.shape[0], width, height,
+                              float(transmittance_threshold), int(max_steps), _lib.ptr(color), _lib.ptr(log_t),
+                              _lib.ptr(grad_color), _lib.ptr(grad_log_t), ctypes.byref(g), _lib.ptr(status),
+                              _lib.stream_ptr())
+    _lib.check(rc, "pf_walk_backward")
+    grads["status"] = status
+    return grads
+
+
+class WalkGradFn(torch.autograd.Function):
+    """Autograd glue of the walk, shaped like RasterGradFn
+    (rasterize.py:23-34, _backward :1865-1984): outputs (color, alpha, depth),
+    gradients for (points, radii, density, normals, texel_sites, texel_rgb,
+    texel_height); non-finite gradients are zeroed (rasterize.py:1958-1962)."""
+
+    @staticmethod
+    def forward(ctx, tracer, cams_dev, width, height, transmittance_threshold, points, radii, density, normals,
+                texel_sites, texel_rgb, texel_height, adjacency, adjacency_offsets, adjacency_diff):
+        scene = WalkScene(points, radii, density, normals, texel_sites, texel_rgb, texel_height, adjacency,
+                          adjacency_offsets, adjacency_diff, attr_dtype=tracer.tscalar)
+        out = launch_walk_forward(scene, cams_dev, width, height, transmittance_threshold,
+                                  max_steps=tracer.max_steps)
+        ctx.scene = scene
+        ctx.cams_dev = cams_dev
+        ctx.dims = (width, height, transmittance_threshold, tracer.max_steps)
+        ctx.save_for_backward(out["color"], out["log_t"])
+        ctx.mark_non_differentiable(out["depth"])
+        alpha = 1.0 - torch.exp(out["log_t"])
+        return out["color"], alpha, out["depth"]
+
+    @staticmethod
+    def backward(ctx, grad_color, grad_alpha, grad_depth):
+        color, log_t = ctx.saved_tensors
+        width, height, thr, max_steps = ctx.dims
+        g_log_t = None
+        if grad_alpha is not None:
+            g_log_t = (-grad_alpha * torch.exp(log_t)).contiguous()
+        gc = grad_color.contiguous() if grad_color is not None else None
+        grads = launch_walk_backward(ctx.scene, ctx.cams_dev, width, height, color, log_t, gc, g_log_t, thr,
+                                     max_steps)
+        for k in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
+            t = grads[k]
+            t[~t.isfinite()] = 0
+        sp, ns = grads["spheres"], grads["nsigmas"]
+        del ctx.scene
+        return (None, None, None, None, None, sp[:, :3], sp[:, 3], ns[:, 3], ns[:, :3], grads["texel_sites"],
+                grads["texel_rgb"], grads["texel_height"], None, None, None)
+
+
+class RayTracer:
+    """Drop-in for powerfoam.raytrace.RayTracer (raytrace.py:15-309)."""
+
+    def __init__(self, args, device, attr_dtype="float"):
+        self.device = device
+        self.args = args
+        if attr_dtype == "float":
+            self.tscalar = torch.float32
+        elif attr_dtype == "half":
+            self.tscalar = torch.float16
+        else:
+            raise ValueError(f"Unsupported attribute dtype: {attr_dtype}")
+        self.attr_dtype = attr_dtype
+        self.max_steps = getattr(args, "max_walk_steps", DEFAULT_MAX_STEPS) if args is not None else DEFAULT_MAX_STEPS
+        _lib.load()
+
+    def _cams(self, cameras, start_point_idx):
+        if not isinstance(cameras, (list, tuple)):
+            cameras = [cameras]
+        w, h = cameras[0].width, cameras[0].height
+        if any(c.width != w or c.height != h for c in cameras):
+            raise ValueError("all cameras of one call must share width and height")
+        if start_point_idx is None:
+            return cameras, None
+        if isinstance(start_point_idx, int):
+            start_point_idx = [start_point_idx]
+        return cameras, list(start_point_idx)
+
+    def benchmark(self, camera, start_point_idx, points, radii, density, normals, texel_sites, texel_rgb,
+                  texel_height, adjacency, adjacency_offsets, adjacency_diff, transmittance_threshold=1e-3):
+        """raytrace.py:247-309: returns color_out (H, W, 3) f32."""
+        scene = WalkScene(points, radii, density, normals, texel_sites, texel_rgb, texel_height, adjacency,
+                          adjacency_offsets, adjacency_diff, attr_dtype=self.tscalar)
+        cams_dev = cameras_to_device([camera], [int(start_point_idx)], points.device)
+        out = launch_walk_forward(scene, cams_dev, camera.width, camera.height, transmittance_threshold,
+                                  max_steps=self.max_steps, want_log_t=False, want_depth=False)
+        return out["color"][0]
+
+    def forward(self, cameras, start_point_idx, points, radii, density, normals, texel_sites, texel_rgb,
+                texel_height, adjacency, adjacency_offsets, adjacency_diff, transmittance_threshold=1e-3):
+        """Differentiable batch render.  start_point_idx: int, list of ints,
+        or None (start cells found on the GPU).  Returns (color (B,H,W,3),
+        alpha (B,H,W), depth (B,H,W))."""
+        cameras, starts = self._cams(cameras, start_point_idx)
+        cams_dev = cameras_to_device(cameras, starts, points.device)
+        if starts is None:
+            start_cells_device(points, radii, cams_dev)
+        return WalkGradFn.apply(self, cams_dev, cameras[0].width, cameras[0].height, transmittance_threshold,
+                                points, radii, density, normals, texel_sites, texel_rgb, texel_height, adjacency,
+                                adjacency_offsets, adjacency_diff)
+
+    def trace_sequences(self, cameras, start_point_idx, points, radii, density, normals, texel_sites, texel_rgb,
+                        texel_height, adjacency, adjacency_offsets, adjacency_diff, transmittance_threshold=1e-3,
+                        seq_cap=0, depth_quantile=0.5):
+        """Forward render that also returns each ray's visit count, sequence
+        hash and (if seq_cap > 0) its first seq_cap visited cells."""
+        cameras, starts = self._cams(cameras, start_point_idx)
+        cams_dev = cameras_to_device(cameras, starts, points.device)
+        if starts is None:
+            start_cells_device(points, radii, cams_dev)
+        scene = WalkScene(points, radii, density, normals, texel_sites, texel_rgb, texel_height, adjacency,
+                          adjacency_offsets, adjacency_diff, attr_dtype=self.tscalar)
+        out = launch_walk_forward(scene, cams_dev, cameras[0].width, cameras[0].height, transmittance_threshold,
+                                  depth_quantile=depth_quantile, max_steps=self.max_steps, want_sequences=True,
+                                  seq_cap=seq_cap)
+        out["cams"] = cams_dev
+        return out

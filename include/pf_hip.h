@@ -1,0 +1,176 @@
+/*
+ * pf_hip.h — C ABI of the MI355X (gfx950) power-diagram ray tracer.
+ *
+ * Every entry point takes plain device pointers, sizes and a hipStream_t
+ * (passed as void*), enqueues its kernels on that stream and returns an int
+ * status (PF_OK = 0).  Nothing here allocates device memory except where a
+ * comment says so; the caller owns every input and output buffer.
+ *
+ * Reference interface replaced by each entry point is cited as file:line of
+ * IsaacSuo/dynamic-powerfoam.  The reference exposes no C ABI: its boundary
+ * is a set of Python classes that launch Warp kernels on torch's current
+ * stream (powerfoam/warp_interop.py:5-24).  The entry points below are the
+ * kernel launches those classes make, one for one.
+ */
+#ifndef PF_HIP_H
+#define PF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PF_OK                 0
+#define PF_ERR_ARG            1   /* bad pointer, size or enum              */
+#define PF_ERR_DTYPE          2   /* unsupported attribute / face dtype     */
+#define PF_ERR_HIP            3   /* a HIP runtime call failed              */
+#define PF_ERR_OVERFLOW       4   /* a caller-sized buffer is too small     */
+
+/* per-ray / per-call status bits written into a caller-provided int       */
+#define PF_STATUS_STEP_CAP    1   /* a walk hit max_steps (reference: none,
+                                     raytrace.py:163 loops forever)         */
+#define PF_STATUS_LIST_CAP    2   /* a tile list exceeded its buffer         */
+
+/* ---- dtypes ------------------------------------------------------------ */
+#define PF_F32 0
+#define PF_F16 1
+
+/* Pinhole camera, the fields of WarpCamera used on the pinhole path
+ * (powerfoam/camera.py:8-17).  start_cell is the walk's first cell
+ * (benchmark.py:320-324); rasterizer entry points ignore it.             */
+typedef struct pf_camera {
+    float eye[3];
+    float right[3];
+    float up[3];
+    int32_t start_cell;
+    int32_t width;
+    int32_t height;
+} pf_camera;                       /* 48 bytes */
+
+/* A resident scene: device pointers in the reference's storage layout
+ * (raytrace.py:274-280 packs spheres = [points, radii] and
+ * nsigmas = [normals, density]; texel arrays are (N,K,3)/(N,K)).          */
+typedef struct pf_scene {
+    const float   *spheres;           /* (N,4) f32                          */
+    const void    *nsigmas;           /* (N,4) attr dtype                   */
+    const void    *texel_sites;       /* (N,K,3) attr dtype                 */
+    const void    *texel_rgb;         /* (N,K,3) attr dtype                 */
+    const void    *texel_height;      /* (N,K) attr dtype                   */
+    const int32_t *adjacency;         /* (E,)   CSR neighbour indices       */
+    const int32_t *adjacency_offsets; /* (N+1,) CSR row offsets             */
+    const void    *faces;             /* (E,4) face dtype: walk faces
+                                         (dp, dpm) or raster faces (dp, r_j) */
+    int64_t num_cells;
+    int64_t num_edges;
+    int32_t num_texel_sites;          /* K, 1..16                           */
+    int32_t attr_dtype;               /* PF_F32 | PF_F16                    */
+    int32_t face_dtype;               /* PF_F32 | PF_F16                    */
+    int32_t reserved;
+    int64_t texel_rgb_cam_stride;     /* elements between cameras' texel_rgb
+                                         (view-dependent colour); 0: shared */
+} pf_scene;
+
+/* Per-ray outputs of the walk.  Any pointer may be NULL except color.
+ * Layout (B,H,W[,3]) row-major, B = number of cameras.                    */
+typedef struct pf_walk_out {
+    float    *color;      /* (B,H,W,3) composited RGB (raytrace.py:241)     */
+    float    *log_t;      /* (B,H,W)   final log transmittance              */
+    float    *depth;      /* (B,H,W)   depth where T crosses depth_quantile
+                             (rasterize.py:871-877 semantics)               */
+    int32_t  *visits;     /* (B,H,W)   number of cells visited               */
+    uint64_t *seq_hash;   /* (B,H,W)   FNV-1a-64 of the visited cell ids     */
+    int32_t  *seq;        /* (B,H,W,seq_cap) first seq_cap visited cells     */
+    int32_t   seq_cap;
+    int32_t   reserved;
+    int32_t  *status;     /* one int, PF_STATUS_* bits OR-ed in (may be NULL)*/
+} pf_walk_out;
+
+/* Gradients produced by the walk backward; all f32, accumulated (+=) so the
+ * caller zeroes them.  Shapes follow RasterGradFn's outputs
+ * (rasterize.py:1911-1915).                                                */
+typedef struct pf_grads {
+    float *spheres;       /* (N,4) d/d[points, radii]                       */
+    float *nsigmas;       /* (N,4) d/d[normals, density]                    */
+    float *texel_sites;   /* (N,K,3)                                        */
+    float *texel_rgb;     /* (N,K,3)                                        */
+    float *texel_height;  /* (N,K)                                          */
+    int64_t texel_rgb_cam_stride; /* elements between cameras' texel_rgb
+                                     gradients; 0: one shared array        */
+} pf_grads;
+
+/* Library / device information.  Returns the HIP device count in *n.      */
+int pf_device_count(int32_t *n);
+/* Human-readable message for a status code (static storage).             */
+const char *pf_status_string(int32_t code);
+
+/* ---- walk (RayTracer) --------------------------------------------------- */
+
+/* Power-cell walk, forward.  Replaces RayTracer.benchmark's launch of
+ * benchmark_kernel (powerfoam/raytrace.py:115-241, launch :289-307) and
+ * extends it to a batch of cameras and the extra per-ray outputs above.
+ * cams: device array of num_cams pf_camera, all of width x height.        */
+int pf_walk_forward(const pf_scene *scene, const pf_camera *cams,
+                    int32_t num_cams, int32_t width, int32_t height,
+                    float transmittance_threshold, float depth_quantile,
+                    int32_t max_steps, const pf_walk_out *out, void *stream);
+
+/* Power-cell walk, backward by checkpoint and replay.  No reference
+ * counterpart: RayTracer defines only a forward benchmark_kernel
+ * (raytrace.py:115-241) and no backward.  The
+ * forward's per-ray color and log_t are the checkpoint; each ray's cell
+ * sequence is replayed from its start cell and the gradients of
+ * L(color, 1-exp(log_t)) are accumulated into grads with wave-aggregated
+ * atomics.  grad_color (B,H,W,3), grad_log_t (B,H,W) (either may be NULL). */
+int pf_walk_backward(const pf_scene *scene, const pf_camera *cams,
+                     int32_t num_cams, int32_t width, int32_t height,
+                     float transmittance_threshold, int32_t max_steps,
+                     const float *color, const float *log_t,
+                     const float *grad_color, const float *grad_log_t,
+                     const pf_grads *grads, int32_t *status, void *stream);
+
+/* Walk face precompute (benchmark.py:295-302): for every CSR edge i->j,
+ * faces[e] = half(p_j - p_i, pm_j - pm_i) with pm = 0.5(|p|^2 - r^2).
+ * face_dtype PF_F16 (reference) or PF_F32.                                 */
+int pf_walk_faces(const float *points, const float *radii, int64_t num_cells,
+                  const int32_t *adjacency, const int32_t *adjacency_offsets,
+                  int64_t num_edges, void *faces, int32_t face_dtype,
+                  void *stream);
+
+/* Start cell (benchmark.py:320-324): for each camera,
+ * argmin_i |p_i - eye|^2 - r_i^2 (first index on ties), written into
+ * cams[b].start_cell on the device.  workspace: caller-provided device
+ * buffer of at least 8*num_cams bytes (contents need not be initialised).  */
+int pf_start_cells(const float *points, const float *radii, int64_t num_cells,
+                   pf_camera *cams, int32_t num_cams, void *workspace,
+                   void *stream);
+
+/* ---- colour pre-pass ---------------------------------------------------- */
+
+/* Spherical-harmonics colour per texel site and camera (degree 0..3,
+ * coeffs (N,(D+1)^2,3) f32, sites (N,K,3) f32):
+ * rgb = max(0, 0.5 + sum_l Y_l(normalize(site - eye)) k_l) with the
+ * constants of powerfoam/sh_utils.py:4-74, the value+0.5/clamp of
+ * powerfoam/color_fn.py:95-98 and its per-texel-site direction (:71).
+ * Replaces the colour pre-pass SphericalVoronoi.forward
+ * (color_fn.py:188-246) for SH-coloured scenes.  Output (B,N,K,3) with
+ * rgb_cam_stride elements between cameras, rgb_dtype PF_F32|PF_F16.       */
+int pf_sh_forward(const float *coeffs, int32_t degree, const float *texel_sites,
+                  int64_t num_cells, int32_t num_texel_sites,
+                  const pf_camera *cams, int32_t num_cams, void *texel_rgb,
+                  int32_t rgb_dtype, int64_t rgb_cam_stride, void *stream);
+
+/* Backward of pf_sh_forward: grad_coeffs += sum over cameras and texels of
+ * Y_l * dL/drgb (where the unclamped value is positive).  One thread per
+ * cell, no atomics.  grad_coeffs (N,(D+1)^2,3) f32.                        */
+int pf_sh_backward(const float *coeffs, int32_t degree, const float *texel_sites,
+                   int64_t num_cells, int32_t num_texel_sites,
+                   const pf_camera *cams, int32_t num_cams,
+                   const float *grad_texel_rgb, int64_t grad_cam_stride,
+                   float *grad_coeffs, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PF_HIP_H */

@@ -1,0 +1,213 @@
+"""ctypes access to the CPU restatement ``oracle/build/libpf_oracle.so``.
+
+TEST INFRASTRUCTURE: only tests/, the smoke check and bench.py's CPU
+baseline use this module, as the checker.  The product path never imports
+it.  Arrays are numpy (host); f16 attribute arrays are widened to f32 by the
+caller (an exact conversion), f16 faces are passed as uint16 bits.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "build", "libpf_oracle.so")
+
+c_void_p, c_int32, c_int64, c_float, c_double = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_float,
+                                                 ctypes.c_double)
+
+
+class SceneIn(ctypes.Structure):
+    _fields_ = [
+        ("spheres", c_void_p), ("nsigmas", c_void_p), ("texel_sites", c_void_p), ("texel_rgb", c_void_p),
+        ("texel_height", c_void_p), ("adjacency", c_void_p), ("offsets", c_void_p), ("faces", c_void_p),
+        ("num_cells", c_int64), ("num_edges", c_int64), ("K", c_int32), ("face_dtype", c_int32),
+    ]
+
+
+_lib = None
+
+
+def build():
+    r = subprocess.run(["make", "-s", "-C", HERE], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
+    return LIB
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        lib = ctypes.CDLL(LIB)
+        lib.orc_num_threads.restype = c_int32
+        lib.orc_start_cell.restype = c_int32
+        lib.orc_walk_forward.restype = c_int32
+        lib.orc_walk_backward.restype = c_int32
+        lib.orc_walk_loss_f64.restype = c_double
+        lib.orc_walk_loss_f64.argtypes = [c_void_p] * 6 + [c_int32, c_int64, c_void_p, c_void_p, c_int64,
+                                                          c_void_p, c_int32, c_double, c_int32, c_void_p, c_int64,
+                                                          c_void_p, c_double, c_int32]
+        lib.orc_walk_forward.argtypes = [c_void_p, c_void_p, c_int32, c_float, c_float, c_int32, c_void_p, c_int64,
+                                         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
+                                         c_int32]
+        lib.orc_walk_backward.argtypes = [c_void_p, c_void_p, c_int32, c_float, c_int32] + [c_void_p] * 8
+        lib.orc_start_cell.argtypes = [c_void_p, c_void_p, c_int64, c_void_p]
+        lib.orc_walk_faces.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int32]
+        lib.orc_ray_dirs.argtypes = [c_void_p, c_void_p]
+        for f in ("orc_f32_to_f16", "orc_f16_to_f32", "orc_expf_vec", "orc_logf_vec"):
+            getattr(lib, f).argtypes = [c_void_p, c_int64, c_void_p]
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def num_threads():
+    return load().orc_num_threads()
+
+
+def f32_to_f16_bits(x):
+    x = _c(x, np.float32)
+    out = np.empty(x.shape, np.uint16)
+    load().orc_f32_to_f16(_p(x), x.size, _p(out))
+    return out
+
+
+def f16_bits_to_f32(x):
+    x = _c(x, np.uint16)
+    out = np.empty(x.shape, np.float32)
+    load().orc_f16_to_f32(_p(x), x.size, _p(out))
+    return out
+
+
+def expf(x):
+    x = _c(x, np.float32)
+    out = np.empty_like(x)
+    load().orc_expf_vec(_p(x), x.size, _p(out))
+    return out
+
+
+def logf(x):
+    x = _c(x, np.float32)
+    out = np.empty_like(x)
+    load().orc_logf_vec(_p(x), x.size, _p(out))
+    return out
+
+
+def ray_dirs(cam_words):
+    cam = _c(cam_words, np.int32)
+    H, W = int(cam[11]), int(cam[10])
+    out = np.empty((H, W, 3), np.float32)
+    load().orc_ray_dirs(_p(cam), _p(out))
+    return out
+
+
+def walk_faces(points, radii, adjacency, offsets, f16=True):
+    points, radii = _c(points, np.float32), _c(radii, np.float32)
+    adjacency, offsets = _c(adjacency, np.int32), _c(offsets, np.int32)
+    out = np.empty((adjacency.size, 4), np.uint16 if f16 else np.float32)
+    load().orc_walk_faces(_p(points), _p(radii), points.shape[0], _p(adjacency), _p(offsets), _p(out),
+                          1 if f16 else 0)
+    return out
+
+
+def start_cell(points, radii, eye):
+    points, radii, eye = _c(points, np.float32), _c(radii, np.float32), _c(eye, np.float32)
+    return int(load().orc_start_cell(_p(points), _p(radii), points.shape[0], _p(eye)))
+
+
+class Scene:
+    """Host scene in the kernels' layout; keeps the numpy arrays alive."""
+
+    def __init__(self, points, radii, density, normals, texel_sites, texel_rgb, texel_height, adjacency, offsets,
+                 faces):
+        self.spheres = _c(np.concatenate([points, radii[:, None]], 1), np.float32)
+        self.nsigmas = _c(np.concatenate([normals, density[:, None]], 1), np.float32)
+        self.texel_sites = _c(texel_sites, np.float32)
+        self.texel_rgb = _c(texel_rgb, np.float32)
+        self.texel_height = _c(texel_height, np.float32)
+        self.adjacency = _c(adjacency, np.int32)
+        self.offsets = _c(offsets, np.int32)
+        self.faces = np.ascontiguousarray(faces)
+        f16 = self.faces.dtype == np.uint16
+        if not f16:
+            self.faces = _c(self.faces, np.float32)
+        s = SceneIn()
+        s.spheres, s.nsigmas = _p(self.spheres), _p(self.nsigmas)
+        s.texel_sites, s.texel_rgb, s.texel_height = _p(self.texel_sites), _p(self.texel_rgb), _p(self.texel_height)
+        s.adjacency, s.offsets, s.faces = _p(self.adjacency), _p(self.offsets), _p(self.faces)
+        s.num_cells = self.spheres.shape[0]
+        s.num_edges = self.adjacency.size
+        s.K = self.texel_sites.shape[1]
+        s.face_dtype = 1 if f16 else 0
+        self.c = s
+        self.K = s.K
+        self.N = s.num_cells
+
+
+def walk_forward(scene, cams, thr=1e-3, depth_q=0.5, max_steps=1 << 16, rays=None, seq_cap=0, nthreads=None):
+    """cams: (B, 12) int32 pf_camera words.  Returns dict of per-ray arrays
+    shaped (B, H, W, ...) or (n_rays, ...) when rays is given."""
+    lib = load()
+    cams = _c(cams, np.int32).reshape(-1, 12)
+    B, W, H = cams.shape[0], int(cams[0, 10]), int(cams[0, 11])
+    if rays is not None:
+        rays = _c(rays, np.int64)
+        shape = (rays.size,)
+    else:
+        shape = (B, H, W)
+    color = np.empty(shape + (3,), np.float32)
+    log_t = np.empty(shape, np.float32)
+    depth = np.empty(shape, np.float32)
+    visits = np.empty(shape, np.int32)
+    hsh = np.empty(shape, np.uint64)
+    seq = np.full(shape + (seq_cap,), -1, np.int32) if seq_cap > 0 else None
+    st = lib.orc_walk_forward(ctypes.byref(scene.c), _p(cams), B, thr, depth_q, max_steps, _p(rays),
+                              0 if rays is None else rays.size, _p(color), _p(log_t), _p(depth), _p(visits), _p(hsh),
+                              _p(seq), seq_cap, nthreads or num_threads())
+    return dict(color=color, log_t=log_t, depth=depth, visits=visits, seq_hash=hsh, seq=seq, status=st)
+
+
+def walk_backward(scene, cams, color, grad_color, grad_log_t=None, thr=1e-3, max_steps=1 << 16):
+    lib = load()
+    cams = _c(cams, np.int32).reshape(-1, 12)
+    n, K = scene.N, scene.K
+    g = dict(spheres=np.zeros((n, 4), np.float32), nsigmas=np.zeros((n, 4), np.float32),
+             texel_sites=np.zeros((n, K, 3), np.float32), texel_rgb=np.zeros((n, K, 3), np.float32),
+             texel_height=np.zeros((n, K), np.float32))
+    color = _c(color, np.float32)
+    gc = None if grad_color is None else _c(grad_color, np.float32)
+    gl = None if grad_log_t is None else _c(grad_log_t, np.float32)
+    st = lib.orc_walk_backward(ctypes.byref(scene.c), _p(cams), cams.shape[0], thr, max_steps, _p(color), _p(gc),
+                               _p(gl), _p(g["spheres"]), _p(g["nsigmas"]), _p(g["texel_sites"]),
+                               _p(g["texel_rgb"]), _p(g["texel_height"]))
+    g["status"] = st
+    return g
+
+
+def walk_loss_f64(params, adjacency, offsets, cams, target, thr=1e-3, w_alpha=0.0, rays=None, max_steps=1 << 16,
+                  nthreads=None):
+    """FP64 forward loss (exact faces) for finite differences.  params: dict
+    of float64 arrays points, radii, nsigmas (N,4), texel_sites, texel_rgb,
+    texel_height."""
+    lib = load()
+    P = {k: _c(v, np.float64) for k, v in params.items()}
+    adjacency, offsets = _c(adjacency, np.int32), _c(offsets, np.int32)
+    cams = _c(cams, np.int32).reshape(-1, 12)
+    target = _c(target, np.float32)
+    K = P["texel_sites"].shape[1]
+    N = P["points"].shape[0]
+    r = None if rays is None else _c(rays, np.int64)
+    return lib.orc_walk_loss_f64(_p(P["points"]), _p(P["radii"]), _p(P["nsigmas"]), _p(P["texel_sites"]),
+                                 _p(P["texel_rgb"]), _p(P["texel_height"]), K, N, _p(adjacency), _p(offsets),
+                                 adjacency.size, _p(cams), cams.shape[0], thr, max_steps, _p(r),
+                                 0 if r is None else r.size, _p(target), w_alpha, nthreads or num_threads())
