@@ -1,0 +1,232 @@
+"""Benchmark: power-cell walk forward + backward on one MI355X.
+
+Workload (the 1M-cell 1080p configuration): 1,000,000 cells with sites from
+an 8-component Gaussian mixture (centres U[-1,1]^3, sigma 0.2), power
+weights U[0, 1e-4] (r = sqrt(w)), density lognormal(0, 1), SH degree 3
+colour (48 coefficients per cell, N(0, 0.1), +0.5 offset), adjacency from the
+lifted-hull power-diagram build, fp16 walk faces, one 1920x1080 pinhole
+camera (60 deg) on the radius-3 sphere, per-pixel L2 loss against a uniform
+random target.
+
+One step = SH colour pre-pass + walk forward + loss gradient + walk backward
+(checkpoint and replay) + SH backward, inputs resident in HBM.  Prints ONE
+JSON line.  Single GPU: --gpus above 1 exits with a message.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dynamic-powerfoam_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cells", type=int, default=1_000_000)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work (0 = skip)")
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays shard over GPUs with the "
+              f"scene replicated, see DESIGN.md; this build runs and reports a single MI355X)", flush=True)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from powerfoam import raytrace as rt
+    from powerfoam import shading, workloads
+    from powerfoam.adjacency import build_power_adjacency
+    from powerfoam.camera import cameras_to_device
+
+    dev = "cuda"
+    t0 = time.time()
+    sc = workloads.gmm_scene(args.cells, seed=args.seed)
+    cam = workloads.sphere_cameras(sc["rng"], 1, args.width, args.height)[0]
+    target_np = sc["rng"].uniform(0.0, 1.0, (1, args.height, args.width, 3)).astype(np.float32)
+    log(f"scene {args.cells} cells generated in {time.time() - t0:.1f}s; building power adjacency (Qhull)")
+    t1 = time.time()
+    adj, off = build_power_adjacency(torch.from_numpy(sc["points"]), torch.from_numpy(sc["radii"]))
+    log(f"adjacency: {adj.numel()} directed edges (mean degree {adj.numel() / args.cells:.2f}) "
+        f"in {time.time() - t1:.1f}s")
+
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    pts, rad = t(sc["points"]), t(sc["radii"])
+    adj_d, off_d = adj.to(dev), off.to(dev)
+    faces = rt.walk_faces(pts, rad, adj_d, off_d, torch.float16)
+    sites = t(sc["texel_sites"])
+    coeffs = t(sc["sh"])
+    cams_dev = cameras_to_device([cam], None, dev)
+    rt.start_cells_device(pts, rad, cams_dev)
+    K = sites.shape[1]
+    texel_rgb = torch.empty(1, args.cells, K, 3, dtype=torch.float32, device=dev)
+    scene = rt.WalkScene(pts, rad, t(sc["density"]), t(sc["normals"]), sites, texel_rgb[0], t(sc["texel_height"]),
+                         adj_d, off_d, faces)
+    target = t(target_np)
+    W, H = args.width, args.height
+    grads = dict(spheres=torch.zeros(args.cells, 4, device=dev), nsigmas=torch.zeros(args.cells, 4, device=dev),
+                 texel_sites=torch.zeros(args.cells, K, 3, device=dev),
+                 texel_rgb=torch.zeros(args.cells, K, 3, device=dev),
+                 texel_height=torch.zeros(args.cells, K, device=dev))
+    g_coeffs = torch.zeros_like(coeffs)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    times = {"fwd": [], "bwd": []}
+
+    def step(record):
+        shading.sh_forward(coeffs, 3, sites, cams_dev, out=texel_rgb)
+        if record:
+            ev[0].record()
+        f = rt.launch_walk_forward(scene, cams_dev, W, H, want_depth=False)
+        if record:
+            ev[1].record()
+        gc = 2.0 * (f["color"] - target)
+        for v in grads.values():
+            v.zero_()
+        g_coeffs.zero_()
+        if record:
+            ev[2].record()
+        b = rt.launch_walk_backward(scene, cams_dev, W, H, f["color"], f["log_t"], gc, None, grads=grads)
+        if record:
+            ev[3].record()
+        shading.sh_backward(coeffs, 3, sites, cams_dev, grads["texel_rgb"][None], grad_coeffs=g_coeffs)
+        return f, b
+
+    log("warmup")
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    # per-kernel durations with events on the launch stream (torch's current)
+    for _ in range(3):
+        step(True)
+        torch.cuda.synchronize()
+        times["fwd"].append(ev[0].elapsed_time(ev[1]))
+        times["bwd"].append(ev[2].elapsed_time(ev[3]))
+    log("timed steps")
+    torch.cuda.synchronize()
+    s0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(False)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - s0) / args.steps
+    rays = W * H
+    mrays = rays / dt / 1e6
+
+    # algorithmic bytes from the kernels' own per-ray counters (untimed pass)
+    shading.sh_forward(coeffs, 3, sites, cams_dev, out=texel_rgb)
+    f = rt.launch_walk_forward(scene, cams_dev, W, H, want_depth=False, want_sequences=True, want_bytes=True)
+    gc = 2.0 * (f["color"] - target)
+    for v in grads.values():
+        v.zero_()
+    b = rt.launch_walk_backward(scene, cams_dev, W, H, f["color"], f["log_t"], gc, None, grads=grads,
+                                want_bytes=True)
+    torch.cuda.synchronize()
+    fwd_bytes = float(f["alg_bytes"].sum().item())
+    bwd_bytes = float(b["alg_bytes"].sum().item())
+    visits = f["visits"].double()
+    status = int(f["status"].item()) | int(b["status"].item())
+    fwd_ms = float(np.median(times["fwd"]))
+    bwd_ms = float(np.median(times["bwd"]))
+    dom = "walk_bwd" if bwd_ms >= fwd_ms else "walk_fwd"
+    dom_bytes, dom_ms = (bwd_bytes, bwd_ms) if dom == "walk_bwd" else (fwd_bytes, fwd_ms)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    log(f"step {dt * 1e3:.2f} ms = {mrays:.1f} Mrays/s; walk fwd {fwd_ms:.2f} ms "
+        f"({fwd_bytes / fwd_ms / 1e6:.0f} GB/s alg), bwd {bwd_ms:.2f} ms ({bwd_bytes / bwd_ms / 1e6:.0f} GB/s alg); "
+        f"visits/ray mean {visits.mean().item():.1f} max {int(visits.max().item())}; status {status}")
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        cpu = cpu_baseline(sc, adj.numpy(), off.numpy(), faces, texel_rgb, cams_dev, f, gc, args.cpu_seconds)
+
+    out = {
+        "metric": "Mrays/s fwd+bwd (1M cells, 1080p) + p50/p99 render latency vs plain path",
+        "value": round(mrays, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (fp16 faces)",
+        "data": "synthetic (seeded Gaussian-mixture scene, random target image)",
+        "config": {
+            "workload": f"walk fwd+bwd, {args.cells} cells GMM, {W}x{H}, SH3, L2 loss",
+            "cells": args.cells,
+            "width": W,
+            "height": H,
+            "rays_per_step": rays,
+            "mean_visits_per_ray": round(visits.mean().item(), 2),
+            "walk_fwd_ms": round(fwd_ms, 4),
+            "walk_bwd_ms": round(bwd_ms, 4),
+            "walk_fwd_alg_bytes": fwd_bytes,
+            "walk_bwd_alg_bytes": bwd_bytes,
+            "status": status,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(sc, adj, off, faces, texel_rgb, cams_dev, fwd, gc, seconds):
+    """The CPU restatement (oracle/) timed on this host's cores: walk forward
+    + backward on a bounded, evenly spaced sample of the same frame's rays."""
+    import numpy as np
+    import torch
+
+    from oracle import oracle as orc
+
+    nth = orc.num_threads()
+    cams = cams_dev.cpu().numpy()
+    W, H = int(cams[0, 10]), int(cams[0, 11])
+    fh = faces.cpu().view(torch.int16).numpy().view(np.uint16)
+    cs = orc.Scene(sc["points"], sc["radii"], sc["density"], sc["normals"], sc["texel_sites"],
+                   texel_rgb[0].cpu().numpy(), sc["texel_height"], adj, off, fh)
+    color = fwd["color"].cpu().numpy()
+    gcol = gc.cpu().numpy()
+
+    def run(n):
+        rays = np.linspace(0, W * H - 1, n).astype(np.int64)
+        t0 = time.perf_counter()
+        orc.walk_forward(cs, cams, rays=rays, nthreads=nth)
+        g = np.zeros_like(gcol).reshape(-1, 3)
+        g[rays] = gcol.reshape(-1, 3)[rays]
+        orc.walk_backward(cs, cams, color, g.reshape(gcol.shape), nthreads=nth)
+        return time.perf_counter() - t0
+
+    probe = 4096
+    tp = run(probe)
+    n = int(min(W * H, max(probe, probe * seconds / max(tp, 1e-6))))
+    log(f"cpu baseline: {nth} threads, probe {probe} rays in {tp:.2f}s; sample {n} rays")
+    tt = run(n)
+    return {"value": round(n / tt / 1e6, 5), "unit": "Mrays/s", "cores": nth, "kind": "port",
+            "sample": f"walk fwd+bwd (C++ restatement, OpenMP) on {n} evenly spaced rays of the same 1080p frame, "
+                      f"{tt:.1f}s; SH pre-pass excluded"}
+
+
+if __name__ == "__main__":
+    main()

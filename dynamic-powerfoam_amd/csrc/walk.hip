@@ -101,12 +101,18 @@ __device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(
 // Texel backward: the semantics of plane_intersection_bwd
 // (powerfoam/texture.py:74-186) with the walk's weights (r^-2 multiply).
 // Returns d/d(plane origin), d/d(normal) and d/d(t_near before the plane).
+// exact == false reproduces the reference's two approximations: texels
+// whose weight is below 1e-3 of the weight sum are skipped
+// (texture.py:118-119, :158-159) and the radius gets no gradient through
+// the texel weights.  exact == true keeps every texel and adds that radius
+// term, d w_k / d r = 2 temp w_k |q - s_k|^2 / r^3, into dL_dr.
 template <typename A, int KT>
 __device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 n, float r,
                                           const A *sites, const A *rgbs, const A *heights, int K,
                                           const TexFwd &fw, f3 dL_drgb, float dL_dtsurf_in,
                                           float *g_sites, float *g_rgb, float *g_height,
-                                          f3 &dL_dc, f3 &dL_dn, float &dL_dtnear) {
+                                          f3 &dL_dc, f3 &dL_dn, float &dL_dtnear, bool exact,
+                                          float &dL_dr) {
     if constexpr (KT > 0) K = KT;
     dL_dc = mk(0, 0, 0);
     dL_dn = mk(0, 0, 0);
@@ -125,7 +131,7 @@ __device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 
         const f3 dv = q - s;
         const float dsq = dot(dv, dv) * inv_r2;
         const float w = pf_expf(-kTemp * dsq);
-        if (w / fw.wsum < 1e-3f) continue;
+        if (!exact && w / fw.wsum < 1e-3f) continue;
         const f3 dw_ds = divs(((2.0f * kTemp) * w) * dv, r2);
         const f3 grgb = divs(w * dL_drgb, fw.wsum);
         atomic_add(g_rgb + 3 * i + 0, grgb.x);
@@ -137,6 +143,7 @@ __device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 
         atomic_add(g_sites + 3 * i + 1, gs.y);
         atomic_add(g_sites + 3 * i + 2, gs.z);
         dL_dq = dL_dq + (-1.0f * gs);
+        if (exact) dL_dr += ((gdp - gdp_in) / fw.wsum) * ((((2.0f * kTemp) * w) * dot(dv, dv)) / (r2 * r));
     }
     const float dL_dtq = dot(dL_dq, d);
     dL_dtnear = (fw.dp >= 0.0f || t_near_in >= fw.t_surf) ? dL_dtq : 0.0f;
@@ -162,7 +169,7 @@ __device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 
         const f3 dv = q0 - s;
         const float dsq = dot(dv, dv) * inv_r2;
         const float w = pf_expf(-kTemp * dsq);
-        if (w / fw.wsum0 < 1e-3f) continue;
+        if (!exact && w / fw.wsum0 < 1e-3f) continue;
         const f3 dw_ds = divs(((2.0f * kTemp) * w) * dv, r2);
         atomic_add(g_height + i, (w * dL_dheight) / fw.wsum0);
         const float hdp = dL_dheight * (float)heights[i];
@@ -171,6 +178,7 @@ __device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 
         atomic_add(g_sites + 3 * i + 1, gs.y);
         atomic_add(g_sites + 3 * i + 2, gs.z);
         dL_dq0 = dL_dq0 + (-1.0f * gs);
+        if (exact) dL_dr += ((hdp - hdp_in) / fw.wsum0) * ((((2.0f * kTemp) * w) * dot(dv, dv)) / (r2 * r));
     }
     const float dL_dtq0 = dot(dL_dq0, d);
     dL_dtnear += (fw.dp >= 0.0f || t_near_in >= fw.t0) ? dL_dtq0 : 0.0f;
@@ -196,6 +204,8 @@ struct WalkParams {
     const float *ck_color, *ck_log_t, *g_color, *g_log_t;
     pf_grads grads;
     int *status;
+    bool exact_texel;
+    uint64_t *alg_bytes;  // backward per-ray byte counter (may be null)
 };
 
 struct TileLane {
@@ -222,8 +232,20 @@ __device__ __forceinline__ TileLane tile_lane(const WalkParams &p, int waves_per
     return tl;
 }
 
+// Algorithmic bytes of the walk (what a visit, a face or a shaded segment
+// must read at least once): sphere 16 B + CSR row bounds 8 B + [n, sigma];
+// per face the neighbour index and the face; per shaded segment K texels
+// of (site, rgb, height).
+template <typename A, typename F> struct WalkBytes {
+    static constexpr uint64_t visit = 16 + 8 + 4 * sizeof(A);
+    static constexpr uint64_t face = 4 + 4 * sizeof(F);
+    static constexpr uint64_t texel = 7 * sizeof(A);
+};
+
 template <typename A, typename F, int KT>
 __global__ __launch_bounds__(256) void walk_fwd_kernel(WalkParams p) {
+    constexpr uint64_t kVisitBytes = WalkBytes<A, F>::visit, kFaceBytes = WalkBytes<A, F>::face,
+                       kTexelBytes = WalkBytes<A, F>::texel;
     const TileLane tl = tile_lane(p, 4);
     if (!tl.valid) return;
     const Cam cam = load_cam(p.cams + 12 * tl.b);
@@ -245,6 +267,7 @@ __global__ __launch_bounds__(256) void walk_fwd_kernel(WalkParams p) {
     int prim = cam.start;
     int visits = 0;
     uint64_t hash = kFnvOffset;
+    uint64_t bytes = 0;
     int st = 0;
 
     while (true) {
@@ -266,6 +289,7 @@ __global__ __launch_bounds__(256) void walk_fwd_kernel(WalkParams p) {
 
         const int e0 = p.s.adjacency_offsets[prim];
         const int e1 = p.s.adjacency_offsets[prim + 1];
+        bytes += kVisitBytes + (uint64_t)(e1 - e0) * kFaceBytes;
         int next = kNoCell;
         float pt_far = 1e10f;
         for (int e = e0; e < e1; ++e) {
@@ -285,6 +309,7 @@ __global__ __launch_bounds__(256) void walk_fwd_kernel(WalkParams p) {
         if (!sh.hit || tn > tf || sigma < 1e-3f) { prim = next; continue; }
 
         const int64_t tbase = (int64_t)prim * K;
+        bytes += (uint64_t)K * kTexelBytes;
         const TexFwd tx = texel_fwd<A, KT>(o, d, tn, c, nrm, r, tsites + 3 * tbase, trgb + 3 * tbase,
                                           theight + tbase, K);
         tf = tx.dp >= 0.0f ? fmin_(tx.t_surf, tf) : tf;
@@ -315,6 +340,7 @@ __global__ __launch_bounds__(256) void walk_fwd_kernel(WalkParams p) {
     if (p.out.depth) p.out.depth[tl.ray] = depth;
     if (p.out.visits) p.out.visits[tl.ray] = visits;
     if (p.out.seq_hash) p.out.seq_hash[tl.ray] = hash;
+    if (p.out.alg_bytes) p.out.alg_bytes[tl.ray] = bytes + 12 + (p.out.log_t ? 4 : 0) + (p.out.depth ? 4 : 0);
     if (st && p.out.status) atomicOr(p.out.status, st);
 }
 
@@ -328,6 +354,8 @@ __global__ __launch_bounds__(256) void walk_fwd_kernel(WalkParams p) {
 // sphere, a power face, or the texel plane mirrors rasterize.py:1296-1516.
 template <typename A, typename F, int KT>
 __global__ __launch_bounds__(256) void walk_bwd_kernel(WalkParams p) {
+    constexpr uint64_t kVisitBytes = WalkBytes<A, F>::visit, kFaceBytes = WalkBytes<A, F>::face,
+                       kTexelBytes = WalkBytes<A, F>::texel;
     const TileLane tl = tile_lane(p, 4);
     if (!tl.valid) return;
     const Cam cam = load_cam(p.cams + 12 * tl.b);
@@ -346,13 +374,17 @@ __global__ __launch_bounds__(256) void walk_bwd_kernel(WalkParams p) {
     const f3 g = p.g_color ? mk(p.g_color[3 * tl.ray], p.g_color[3 * tl.ray + 1], p.g_color[3 * tl.ray + 2])
                            : mk(0, 0, 0);
     const float gL = p.g_log_t ? p.g_log_t[tl.ray] : 0.0f;
-    if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f && gL == 0.0f) return;
+    if (g.x == 0.0f && g.y == 0.0f && g.z == 0.0f && gL == 0.0f) {
+        if (p.alg_bytes) p.alg_bytes[tl.ray] = 12 + (p.g_log_t ? 4 : 0);
+        return;
+    }
 
     f3 rgb = mk(0, 0, 0);
     float log_t = 0.0f;
     int prim = cam.start;
     int visits = 0;
     int st = 0;
+    uint64_t bytes = 12 + 12 + (p.g_log_t ? 4 : 0);  // checkpoint colour + incoming gradients
 
     while (true) {
         const float trans = pf_expf(log_t);
@@ -372,6 +404,7 @@ __global__ __launch_bounds__(256) void walk_bwd_kernel(WalkParams p) {
 
         const int e0 = p.s.adjacency_offsets[prim];
         const int e1 = p.s.adjacency_offsets[prim + 1];
+        bytes += kVisitBytes + (uint64_t)(e1 - e0) * kFaceBytes;
         const int n_adj = e1 - e0;
         int next = kNoCell;
         float pt_far = 1e10f;
@@ -394,6 +427,7 @@ __global__ __launch_bounds__(256) void walk_bwd_kernel(WalkParams p) {
         if (!sh.hit || tn > tf || sigma < 1e-3f) { prim = next; continue; }
 
         const int64_t tbase = (int64_t)prim * K;
+        bytes += (uint64_t)K * kTexelBytes;
         const float tn0 = tn;
         const int tn0_id = tn_id, tf0_id = tf_id;
         const TexFwd tx = texel_fwd<A, KT>(o, d, tn, c, nrm, r, tsites + 3 * tbase, trgb + 3 * tbase,
@@ -415,6 +449,7 @@ __global__ __launch_bounds__(256) void walk_bwd_kernel(WalkParams p) {
         rgb.z = __builtin_fmaf(tx.rgb.z * alpha, trans, rgb.z);
         log_t += dlt;
 
+        bytes += 2ull * (32ull + (uint64_t)K * 28ull);  // read-modify-write of the cell's gradients
         const float dL_dLnext = dot(g, Cfin - rgb) + gL;
         const float dL_dalpha = dot(g, tx.rgb) * trans;
         const f3 dL_dcolor = (g * trans) * alpha;
@@ -428,13 +463,13 @@ __global__ __launch_bounds__(256) void walk_bwd_kernel(WalkParams p) {
         else if (tf_id == n_adj) dL_dts = dL_dtf;
 
         f3 dL_dc, dL_dn;
-        float dL_dtn0;
+        float dL_dtn0, dL_dr_tex = 0.0f;
         texel_bwd<A, KT>(o, d, tn0, c, nrm, r, tsites + 3 * tbase, trgb + 3 * tbase, theight + tbase, K, tx,
                          dL_dcolor, dL_dts, p.grads.texel_sites + 3 * tbase,
                          p.grads.texel_rgb + (int64_t)tl.b * p.grads.texel_rgb_cam_stride + 3 * tbase,
-                         p.grads.texel_height + tbase, dL_dc, dL_dn, dL_dtn0);
+                         p.grads.texel_height + tbase, dL_dc, dL_dn, dL_dtn0, p.exact_texel, dL_dr_tex);
         if (tn_id != n_adj) dL_dtn0 += dL_dtn;
-        float dL_dr = 0.0f;
+        float dL_dr = dL_dr_tex;
         for (int side = 0; side < 2; ++side) {
             const int tid = side == 0 ? tn0_id : tf0_id;
             const float dLdt = side == 0 ? dL_dtn0 : (tf_id != n_adj ? dL_dtf : 0.0f);
@@ -458,6 +493,7 @@ __global__ __launch_bounds__(256) void walk_bwd_kernel(WalkParams p) {
                 atomic_add(ga + 1, dLdt * fg.dcj.y);
                 atomic_add(ga + 2, dLdt * fg.dcj.z);
                 atomic_add(ga + 3, dLdt * fg.drj);
+                bytes += 32;  // neighbour sphere gradient read-modify-write
             }
         }
         float *gs = p.grads.spheres + 4 * (int64_t)cell;
@@ -471,6 +507,7 @@ __global__ __launch_bounds__(256) void walk_bwd_kernel(WalkParams p) {
         atomic_add(gn + 2, dL_dn.z);
         atomic_add(gn + 3, dL_dsigma);
     }
+    if (p.alg_bytes) p.alg_bytes[tl.ray] = bytes;
     if (st && p.status) atomicOr(p.status, st);
 }
 
@@ -613,7 +650,8 @@ extern "C" int pf_walk_forward(const pf_scene *scene, const pf_camera *cams, int
 extern "C" int pf_walk_backward(const pf_scene *scene, const pf_camera *cams, int32_t num_cams, int32_t width,
                                 int32_t height, float transmittance_threshold, int32_t max_steps, const float *color,
                                 const float *log_t, const float *grad_color, const float *grad_log_t,
-                                const pf_grads *grads, int32_t *status, void *stream) {
+                                const pf_grads *grads, int32_t flags, int32_t *status, uint64_t *alg_bytes,
+                                void *stream) {
     int rc = check_scene(scene);
     if (rc) return rc;
     if (!cams || num_cams <= 0 || width <= 0 || height <= 0 || !color || !grads || max_steps <= 0) return PF_ERR_ARG;
@@ -627,6 +665,8 @@ extern "C" int pf_walk_backward(const pf_scene *scene, const pf_camera *cams, in
     p.g_log_t = grad_log_t;
     p.grads = *grads;
     p.status = status;
+    p.exact_texel = (flags & PF_BWD_EXACT_TEXEL) != 0;
+    p.alg_bytes = alg_bytes;
     return dispatch_walk(p, true, (hipStream_t)stream);
 }
 

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libpf_hip.so")
 PF_OK, PF_ERR_ARG, PF_ERR_DTYPE, PF_ERR_HIP, PF_ERR_OVERFLOW = 0, 1, 2, 3, 4
 PF_STATUS_STEP_CAP, PF_STATUS_LIST_CAP = 1, 2
 PF_F32, PF_F16 = 0, 1
+PF_BWD_EXACT_TEXEL = 1
 
 c_void_p = ctypes.c_void_p
 c_int32 = ctypes.c_int32
@@ -67,6 +68,7 @@ class PfWalkOut(ctypes.Structure):
         ("seq_cap", c_int32),
         ("reserved", c_int32),
         ("status", c_void_p),
+        ("alg_bytes", c_void_p),
     ]
 
 
@@ -93,7 +95,7 @@ _SIGS = {
     "pf_walk_backward": (
         c_int32,
         [ctypes.POINTER(PfScene), c_void_p, c_int32, c_int32, c_int32, c_float, c_int32, c_void_p, c_void_p,
-         c_void_p, c_void_p, ctypes.POINTER(PfGrads), c_void_p, c_void_p],
+         c_void_p, c_void_p, ctypes.POINTER(PfGrads), c_int32, c_void_p, c_void_p, c_void_p],
     ),
     "pf_walk_faces": (
         c_int32,

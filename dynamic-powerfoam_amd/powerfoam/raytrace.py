@@ -106,7 +106,7 @@ def walk_faces(points, radii, adjacency, adjacency_offsets, dtype=torch.float16)
 
 def launch_walk_forward(scene, cams_dev, width, height, transmittance_threshold=1e-3, depth_quantile=0.5,
                         max_steps=DEFAULT_MAX_STEPS, want_log_t=True, want_depth=True, want_sequences=False,
-                        seq_cap=0):
+                        seq_cap=0, want_bytes=False):
     lib = _lib.load()
     B = cams_dev.shape[0]
     dev = cams_dev.device
@@ -129,15 +129,19 @@ def launch_walk_forward(scene, cams_dev, width, height, transmittance_threshold=
     out.seq = _lib.ptr(seq)
     out.seq_cap = seq_cap if seq is not None else 0
     out.status = _lib.ptr(status)
+    alg_bytes = torch.zeros(B, height, width, dtype=torch.int64, device=dev) if want_bytes else None
+    out.alg_bytes = _lib.ptr(alg_bytes)
     rc = lib.pf_walk_forward(ctypes.byref(scene.c), _lib.ptr(cams_dev), B, width, height,
                              float(transmittance_threshold), float(depth_quantile), int(max_steps),
                              ctypes.byref(out), _lib.stream_ptr())
     _lib.check(rc, "pf_walk_forward")
-    return dict(color=color, log_t=log_t, depth=depth, visits=visits, seq_hash=seq_hash, seq=seq, status=status)
+    return dict(color=color, log_t=log_t, depth=depth, visits=visits, seq_hash=seq_hash, seq=seq, status=status,
+                alg_bytes=alg_bytes)
 
 
 def launch_walk_backward(scene, cams_dev, width, height, color, log_t, grad_color, grad_log_t,
-                         transmittance_threshold=1e-3, max_steps=DEFAULT_MAX_STEPS, grads=None):
+                         transmittance_threshold=1e-3, max_steps=DEFAULT_MAX_STEPS, grads=None, exact_texel=False,
+                         texel_rgb_cam_stride=0, want_bytes=False):
     lib = _lib.load()
     n, K = scene.num_cells, scene.K
     dev = cams_dev.device
@@ -152,13 +156,17 @@ def launch_walk_backward(scene, cams_dev, width, height, color, log_t, grad_colo
     g = _lib.PfGrads()
     for k in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
         setattr(g, k, _lib.ptr(grads[k]))
+    g.texel_rgb_cam_stride = texel_rgb_cam_stride
     status = torch.zeros(1, dtype=torch.int32, device=dev)
+    alg_bytes = torch.zeros(color.shape[:-1], dtype=torch.int64, device=dev) if want_bytes else None
     rc = lib.pf_walk_backward(ctypes.byref(scene.c), _lib.ptr(cams_dev), cams_dev.shape[0], width, height,
                               float(transmittance_threshold), int(max_steps), _lib.ptr(color), _lib.ptr(log_t),
-                              _lib.ptr(grad_color), _lib.ptr(grad_log_t), ctypes.byref(g), _lib.ptr(status),
+                              _lib.ptr(grad_color), _lib.ptr(grad_log_t), ctypes.byref(g),
+                              _lib.PF_BWD_EXACT_TEXEL if exact_texel else 0, _lib.ptr(status), _lib.ptr(alg_bytes),
                               _lib.stream_ptr())
     _lib.check(rc, "pf_walk_backward")
     grads["status"] = status
+    grads["alg_bytes"] = alg_bytes
     return grads
 
 
@@ -178,6 +186,7 @@ class WalkGradFn(torch.autograd.Function):
         ctx.scene = scene
         ctx.cams_dev = cams_dev
         ctx.dims = (width, height, transmittance_threshold, tracer.max_steps)
+        ctx.exact_texel = tracer.exact_texel
         ctx.save_for_backward(out["color"], out["log_t"])
         ctx.mark_non_differentiable(out["depth"])
         alpha = 1.0 - torch.exp(out["log_t"])
@@ -192,7 +201,7 @@ class WalkGradFn(torch.autograd.Function):
             g_log_t = (-grad_alpha * torch.exp(log_t)).contiguous()
         gc = grad_color.contiguous() if grad_color is not None else None
         grads = launch_walk_backward(ctx.scene, ctx.cams_dev, width, height, color, log_t, gc, g_log_t, thr,
-                                     max_steps)
+                                     max_steps, exact_texel=ctx.exact_texel)
         for k in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
             t = grads[k]
             t[~t.isfinite()] = 0
@@ -216,6 +225,8 @@ class RayTracer:
             raise ValueError(f"Unsupported attribute dtype: {attr_dtype}")
         self.attr_dtype = attr_dtype
         self.max_steps = getattr(args, "max_walk_steps", DEFAULT_MAX_STEPS) if args is not None else DEFAULT_MAX_STEPS
+        # False: the reference's texel-gradient approximations (texture.py:118-119)
+        self.exact_texel = bool(getattr(args, "exact_texel_grad", False)) if args is not None else False
         _lib.load()
 
     def _cams(self, cameras, start_point_idx):

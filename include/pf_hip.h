@@ -85,6 +85,9 @@ typedef struct pf_walk_out {
     int32_t   seq_cap;
     int32_t   reserved;
     int32_t  *status;     /* one int, PF_STATUS_* bits OR-ed in (may be NULL)*/
+    uint64_t *alg_bytes;  /* (B,H,W) algorithmic bytes the ray needed: per
+                             visit sphere+CSR bounds+[n,sigma] and per face
+                             index+face; per shaded segment K texels     */
 } pf_walk_out;
 
 /* Gradients produced by the walk backward; all f32, accumulated (+=) so the
@@ -116,19 +119,28 @@ int pf_walk_forward(const pf_scene *scene, const pf_camera *cams,
                     float transmittance_threshold, float depth_quantile,
                     int32_t max_steps, const pf_walk_out *out, void *stream);
 
+/* flags of pf_walk_backward */
+#define PF_BWD_EXACT_TEXEL    1   /* keep every texel and the radius term of
+                                     the texel weights (default: the
+                                     reference's approximations,
+                                     texture.py:118-119, :158-159)          */
+
 /* Power-cell walk, backward by checkpoint and replay.  No reference
  * counterpart: RayTracer defines only a forward benchmark_kernel
  * (raytrace.py:115-241) and no backward.  The
  * forward's per-ray color and log_t are the checkpoint; each ray's cell
  * sequence is replayed from its start cell and the gradients of
  * L(color, 1-exp(log_t)) are accumulated into grads with wave-aggregated
- * atomics.  grad_color (B,H,W,3), grad_log_t (B,H,W) (either may be NULL). */
+ * atomics.  grad_color (B,H,W,3), grad_log_t (B,H,W) (either may be NULL).
+ * alg_bytes (B,H,W) or NULL: per-ray algorithmic bytes (the replay's reads
+ * plus the gradient read-modify-writes).                                   */
 int pf_walk_backward(const pf_scene *scene, const pf_camera *cams,
                      int32_t num_cams, int32_t width, int32_t height,
                      float transmittance_threshold, int32_t max_steps,
                      const float *color, const float *log_t,
                      const float *grad_color, const float *grad_log_t,
-                     const pf_grads *grads, int32_t *status, void *stream);
+                     const pf_grads *grads, int32_t flags, int32_t *status,
+                     uint64_t *alg_bytes, void *stream);
 
 /* Walk face precompute (benchmark.py:295-302): for every CSR edge i->j,
  * faces[e] = half(p_j - p_i, pm_j - pm_i) with pm = 0.5(|p|^2 - r^2).

@@ -155,3 +155,31 @@ def render(scene, cam_words, thr=1e-3, depth_q=0.5, pixels=None):
         out["seq"].append(vis)
         out["margin"][k] = m
     return out
+
+
+# Real spherical harmonics, degree <= 3, in FP64 with the constants and
+# signs of powerfoam/sh_utils.py:4-74 (the 3DGS ordering).
+SH_C0 = 0.28209479177387814
+SH_C1 = 0.4886025119029199
+SH_C2 = (1.0925484305920792, -1.0925484305920792, 0.31539156525252005, -1.0925484305920792, 0.5462742152960396)
+SH_C3 = (-0.5900435899266435, 2.890611442640554, -0.4570457994644658, 0.3731763325901154, -0.4570457994644658,
+         1.445305721320277, -0.5900435899266435)
+
+
+def sh_basis_fp64(v, degree):
+    """Basis values at the directions v / |v|: (..., (degree+1)^2)."""
+    v = np.asarray(v, np.float64)
+    v = v / np.linalg.norm(v, axis=-1, keepdims=True)
+    x, y, z = v[..., 0], v[..., 1], v[..., 2]
+    Y = [np.full(x.shape, SH_C0)]
+    if degree >= 1:
+        Y += [-SH_C1 * y, SH_C1 * z, -SH_C1 * x]
+    if degree >= 2:
+        xx, yy, zz = x * x, y * y, z * z
+        Y += [SH_C2[0] * x * y, SH_C2[1] * y * z, SH_C2[2] * (2 * zz - xx - yy), SH_C2[3] * x * z,
+              SH_C2[4] * (xx - yy)]
+    if degree >= 3:
+        Y += [SH_C3[0] * y * (3 * xx - yy), SH_C3[1] * x * y * z, SH_C3[2] * y * (4 * zz - xx - yy),
+              SH_C3[3] * z * (2 * zz - 3 * xx - 3 * yy), SH_C3[4] * x * (4 * zz - xx - yy), SH_C3[5] * z * (xx - yy),
+              SH_C3[6] * x * (xx - 3 * yy)]
+    return np.stack(Y, -1)

@@ -53,7 +53,7 @@ def load():
         lib.orc_walk_forward.argtypes = [c_void_p, c_void_p, c_int32, c_float, c_float, c_int32, c_void_p, c_int64,
                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
                                          c_int32]
-        lib.orc_walk_backward.argtypes = [c_void_p, c_void_p, c_int32, c_float, c_int32] + [c_void_p] * 8
+        lib.orc_walk_backward.argtypes = [c_void_p, c_void_p, c_int32, c_float, c_int32] + [c_void_p] * 8 + [c_int32, c_int32]
         lib.orc_start_cell.argtypes = [c_void_p, c_void_p, c_int64, c_void_p]
         lib.orc_walk_faces.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int32]
         lib.orc_ray_dirs.argtypes = [c_void_p, c_void_p]
@@ -177,7 +177,9 @@ def walk_forward(scene, cams, thr=1e-3, depth_q=0.5, max_steps=1 << 16, rays=Non
     return dict(color=color, log_t=log_t, depth=depth, visits=visits, seq_hash=hsh, seq=seq, status=st)
 
 
-def walk_backward(scene, cams, color, grad_color, grad_log_t=None, thr=1e-3, max_steps=1 << 16):
+def walk_backward(scene, cams, color, grad_color, grad_log_t=None, thr=1e-3, max_steps=1 << 16, exact=False,
+                  nthreads=1):
+    """nthreads 1: fixed summation order; > 1: atomic adds (timing only)."""
     lib = load()
     cams = _c(cams, np.int32).reshape(-1, 12)
     n, K = scene.N, scene.K
@@ -189,7 +191,8 @@ def walk_backward(scene, cams, color, grad_color, grad_log_t=None, thr=1e-3, max
     gl = None if grad_log_t is None else _c(grad_log_t, np.float32)
     st = lib.orc_walk_backward(ctypes.byref(scene.c), _p(cams), cams.shape[0], thr, max_steps, _p(color), _p(gc),
                                _p(gl), _p(g["spheres"]), _p(g["nsigmas"]), _p(g["texel_sites"]),
-                               _p(g["texel_rgb"]), _p(g["texel_height"]))
+                               _p(g["texel_rgb"]), _p(g["texel_height"]), 1 if exact else 0,
+                               nthreads)
     g["status"] = st
     return g
 

@@ -112,6 +112,18 @@ inline uint16_t float_to_half(float f) {
     return (uint16_t)(s | r);
 }
 
+// Gradient accumulation: plain += (single thread, fixed order) or an atomic
+// add when rays run on several threads (the CPU baseline's backward).
+thread_local bool g_atomic_acc = false;
+template <typename R> inline void acc(R &dst, R v) {
+    if (g_atomic_acc) {
+#pragma omp atomic
+        dst += v;
+    } else {
+        dst += v;
+    }
+}
+
 template <typename R> struct V3 { R x, y, z; };
 template <typename R> inline V3<R> mk(R x, R y, R z) { return V3<R>{x, y, z}; }
 template <typename R> inline V3<R> sub(V3<R> a, V3<R> b) { return mk<R>(a.x - b.x, a.y - b.y, a.z - b.z); }
@@ -294,7 +306,7 @@ Tex<R> texel_fwd(V3<R> o, V3<R> d, R t_near, V3<R> c, V3<R> n, R r, const R *sit
 template <typename R>
 void texel_bwd(V3<R> o, V3<R> d, R tn_in, V3<R> c, V3<R> n, R r, const R *sites, const R *rgbs, const R *heights,
                int K, const Tex<R> &fw, V3<R> g_rgb_in, R dts_in, R *g_sites, R *g_rgb, R *g_height, V3<R> &dc,
-               V3<R> &dn, R &dtn) {
+               V3<R> &dn, R &dtn, bool exact, R &dr) {
     dc = dn = mk<R>(0, 0, 0);
     R dh = 0;
     const R inv_r2 = R(1) / (r * r);
@@ -306,18 +318,19 @@ void texel_bwd(V3<R> o, V3<R> d, R tn_in, V3<R> c, V3<R> n, R r, const R *sites,
     for (int i = 0; i < K; ++i) {
         const V3<R> dvv = sub(q, mk(sites[3 * i], sites[3 * i + 1], sites[3 * i + 2]));
         const R w = xexp(-R(10) * (dot(dvv, dvv) * inv_r2));
-        if (w / fw.wsum < (R)1e-3f) continue;
+        if (!exact && w / fw.wsum < (R)1e-3f) continue;
         const V3<R> dwds = dv(scl((R(2) * R(10)) * w, dvv), r2);
         const V3<R> grgb = dv(scl(w, g_rgb_in), fw.wsum);
-        g_rgb[3 * i] += grgb.x;
-        g_rgb[3 * i + 1] += grgb.y;
-        g_rgb[3 * i + 2] += grgb.z;
+        acc(g_rgb[3 * i], grgb.x);
+        acc(g_rgb[3 * i + 1], grgb.y);
+        acc(g_rgb[3 * i + 2], grgb.z);
         const R gdp = dot(g_rgb_in, mk(rgbs[3 * i], rgbs[3 * i + 1], rgbs[3 * i + 2]));
         const V3<R> gs = dv(sclr(dwds, gdp - gdp_in), fw.wsum);
-        g_sites[3 * i] += gs.x;
-        g_sites[3 * i + 1] += gs.y;
-        g_sites[3 * i + 2] += gs.z;
+        acc(g_sites[3 * i], gs.x);
+        acc(g_sites[3 * i + 1], gs.y);
+        acc(g_sites[3 * i + 2], gs.z);
         dq = add(dq, scl(R(-1), gs));
+        if (exact) dr += ((gdp - gdp_in) / fw.wsum) * ((((R(2) * R(10)) * w) * dot(dvv, dvv)) / (r2 * r));
     }
     const R dtq = dot(dq, d);
     dtn = (fw.dp >= R(0) || tn_in >= fw.t_surf) ? dtq : R(0);
@@ -338,15 +351,16 @@ void texel_bwd(V3<R> o, V3<R> d, R tn_in, V3<R> c, V3<R> n, R r, const R *sites,
     for (int i = 0; i < K; ++i) {
         const V3<R> dvv = sub(q0, mk(sites[3 * i], sites[3 * i + 1], sites[3 * i + 2]));
         const R w = xexp(-R(10) * (dot(dvv, dvv) * inv_r2));
-        if (w / fw.wsum0 < (R)1e-3f) continue;
+        if (!exact && w / fw.wsum0 < (R)1e-3f) continue;
         const V3<R> dwds = dv(scl((R(2) * R(10)) * w, dvv), r2);
-        g_height[i] += (w * dh) / fw.wsum0;
+        acc(g_height[i], (w * dh) / fw.wsum0);
         const R hdp = dh * heights[i];
         const V3<R> gs = dv(sclr(dwds, hdp - hdp_in), fw.wsum0);
-        g_sites[3 * i] += gs.x;
-        g_sites[3 * i + 1] += gs.y;
-        g_sites[3 * i + 2] += gs.z;
+        acc(g_sites[3 * i], gs.x);
+        acc(g_sites[3 * i + 1], gs.y);
+        acc(g_sites[3 * i + 2], gs.z);
         dq0 = add(dq0, scl(R(-1), gs));
+        if (exact) dr += ((hdp - hdp_in) / fw.wsum0) * ((((R(2) * R(10)) * w) * dot(dvv, dvv)) / (r2 * r));
     }
     const R dtq0 = dot(dq0, d);
     dtn += (fw.dp >= R(0) || tn_in >= fw.t0) ? dtq0 : R(0);
@@ -446,7 +460,7 @@ RayOut<R> walk_ray(const SceneR<R> &s, FaceLoad face, V3<R> o, V3<R> d, int star
 // per-segment routing mirrors rasterize.py:1296-1516.
 template <typename R, typename FaceLoad>
 int walk_ray_bwd(const SceneR<R> &s, FaceLoad face, V3<R> o, V3<R> d, int start, R thr, int max_steps, V3<R> Cfin,
-                 V3<R> g, R gL, R *gsph, R *gns, R *gsite, R *grgb, R *gh) {
+                 V3<R> g, R gL, R *gsph, R *gns, R *gsite, R *grgb, R *gh, bool exact) {
     R log_t = 0;
     V3<R> rgb = mk<R>(0, 0, 0);
     int prim = start, visits = 0;
@@ -513,11 +527,11 @@ int walk_ray_bwd(const SceneR<R> &s, FaceLoad face, V3<R> o, V3<R> d, int start,
         if (tn_id == n_adj) dts = dtn;
         else if (tf_id == n_adj) dts = dtf;
         V3<R> dc, dn;
-        R dtn0;
+        R dtn0, dr_tex = 0;
         texel_bwd(o, d, tn0, c, nrm, r, s.sites + 3 * tb, s.rgb + 3 * tb, s.height + tb, s.K, tx, dcolor, dts,
-                  gsite + 3 * tb, grgb + 3 * tb, gh + tb, dc, dn, dtn0);
+                  gsite + 3 * tb, grgb + 3 * tb, gh + tb, dc, dn, dtn0, exact, dr_tex);
         if (tn_id != n_adj) dtn0 += dtn;
-        R dr = 0;
+        R dr = dr_tex;
         for (int side = 0; side < 2; ++side) {
             const int tid = side == 0 ? tn0_id : tf0_id;
             const R dLdt = side == 0 ? dtn0 : (tf_id != n_adj ? dtf : R(0));
@@ -536,22 +550,22 @@ int walk_ray_bwd(const SceneR<R> &s, FaceLoad face, V3<R> o, V3<R> d, int start,
                 dc = add(dc, sclr(fdc, dLdt));
                 dr += dLdt * fdr;
                 R *ga = gsph + 4 * (int64_t)a;
-                ga[0] += dLdt * fdcj.x;
-                ga[1] += dLdt * fdcj.y;
-                ga[2] += dLdt * fdcj.z;
-                ga[3] += dLdt * fdrj;
+                acc(ga[0], dLdt * fdcj.x);
+                acc(ga[1], dLdt * fdcj.y);
+                acc(ga[2], dLdt * fdcj.z);
+                acc(ga[3], dLdt * fdrj);
             }
         }
         R *gs = gsph + 4 * (int64_t)cell;
-        gs[0] += dc.x;
-        gs[1] += dc.y;
-        gs[2] += dc.z;
-        gs[3] += dr;
+        acc(gs[0], dc.x);
+        acc(gs[1], dc.y);
+        acc(gs[2], dc.z);
+        acc(gs[3], dr);
         R *gn = gns + 4 * (int64_t)cell;
-        gn[0] += dn.x;
-        gn[1] += dn.y;
-        gn[2] += dn.z;
-        gn[3] += dsigma;
+        acc(gn[0], dn.x);
+        acc(gn[1], dn.y);
+        acc(gn[2], dn.z);
+        acc(gn[3], dsigma);
     }
     return 0;
 }
@@ -689,12 +703,16 @@ int orc_walk_forward(const SceneIn *scene, const CamIn *cams, int32_t B, float t
 // fixed order.
 int orc_walk_backward(const SceneIn *scene, const CamIn *cams, int32_t B, float thr, int32_t max_steps,
                       const float *color, const float *grad_color, const float *grad_log_t, float *g_spheres,
-                      float *g_nsigmas, float *g_sites, float *g_rgb, float *g_height) {
+                      float *g_nsigmas, float *g_sites, float *g_rgb, float *g_height, int32_t flags,
+                      int32_t nthreads) {
     const SceneR<float> s = view_f32(*scene);
     const FaceLoadF32 fl{scene};
     const int W = cams[0].width, H = cams[0].height;
     int status = 0;
+    const bool par = nthreads > 1;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(par ? nthreads : 1) reduction(| : status)
     for (int64_t ray = 0; ray < (int64_t)B * H * W; ++ray) {
+        g_atomic_acc = par;
         const int b = (int)(ray / ((int64_t)H * W));
         const int i = (int)((ray / W) % H);
         const int j = (int)(ray % W);
@@ -706,7 +724,7 @@ int orc_walk_backward(const SceneIn *scene, const CamIn *cams, int32_t B, float 
         const V3<float> d = ray_dir(c, i, j);
         status |= walk_ray_bwd(s, fl, c.eye, d, c.start, thr, max_steps,
                                mk(color[3 * ray], color[3 * ray + 1], color[3 * ray + 2]), g, gL, g_spheres, g_nsigmas,
-                               g_sites, g_rgb, g_height);
+                               g_sites, g_rgb, g_height, (flags & 1) != 0);
     }
     return status;
 }

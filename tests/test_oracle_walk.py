@@ -217,7 +217,7 @@ def test_walk_backward_matches_fp64_central_differences(scene_textured):
     w_alpha = 0.3
     gc = 2.0 * (fwd["color"] - target)
     gl = -w_alpha * np.exp(fwd["log_t"])  # d/dlog_t of w_alpha (1 - exp(log_t))
-    g = orc.walk_backward(S, w, fwd["color"], gc, gl)
+    g = orc.walk_backward(S, w, fwd["color"], gc, gl, exact=True)
     assert g["status"] == 0
     P = dict(points=sc["points"].astype(np.float64), radii=sc["radii"].astype(np.float64),
              nsigmas=np.concatenate([sc["normals"], sc["density"][:, None]], 1).astype(np.float64),
