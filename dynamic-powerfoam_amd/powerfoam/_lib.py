@@ -83,6 +83,29 @@ class PfGrads(ctypes.Structure):
     ]
 
 
+class PfRasterOut(ctypes.Structure):
+    _fields_ = [(k, c_void_p) for k in ("color", "log_t", "normal_distance", "normal", "quantile_depths", "err",
+                                         "contrib", "point_err")]
+
+
+class PfRasterGradIn(ctypes.Structure):
+    _fields_ = [(k, c_void_p) for k in ("color", "log_t", "normal_distance", "normal", "quantile_depths",
+                                         "contrib")]
+
+
+class PfVisOptions(ctypes.Structure):
+    _fields_ = [("transmittance_threshold", c_float), ("max_intersections", c_int32), ("depth_quantile", c_float),
+                ("bkgd_color", c_float * 3)]
+
+
+class PfVisOut(ctypes.Structure):
+    _fields_ = [(k, c_void_p) for k in ("color", "depth", "normal", "alpha", "intersections")]
+
+
+class PfCechGrid(ctypes.Structure):
+    _fields_ = [("origin", c_float * 3), ("cell", c_float), ("dims", c_int32 * 3), ("reserved", c_int32)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/pf_hip.h
 _SIGS = {
     "pf_device_count": (c_int32, [ctypes.POINTER(c_int32)]),
@@ -102,6 +125,44 @@ _SIGS = {
         [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p, c_int32, c_void_p],
     ),
     "pf_start_cells": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_int32, c_void_p, c_void_p]),
+    "pf_raster_faces": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
+    "pf_raster_bin_count": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    "pf_raster_scan": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p]),
+    "pf_raster_bin_write": (
+        c_int32,
+        [c_void_p, c_int64, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
+         c_void_p],
+    ),
+    "pf_raster_forward": (
+        c_int32,
+        [ctypes.POINTER(PfScene), c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_float, c_void_p,
+         c_int32, c_void_p, c_int32, c_int32, ctypes.POINTER(PfRasterOut), c_void_p],
+    ),
+    "pf_raster_backward": (
+        c_int32,
+        [ctypes.POINTER(PfScene), c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
+         c_void_p, ctypes.POINTER(PfRasterGradIn), ctypes.POINTER(PfGrads), c_void_p],
+    ),
+    "pf_raster_benchmark": (
+        c_int32,
+        [ctypes.POINTER(PfScene), c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_float, c_void_p, c_void_p],
+    ),
+    "pf_raster_visualize": (
+        c_int32,
+        [ctypes.POINTER(PfScene), c_void_p, c_int32, c_int32, c_void_p, c_void_p, ctypes.POINTER(PfVisOptions),
+         ctypes.POINTER(PfVisOut), c_void_p],
+    ),
+    "pf_cech_bounds": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    "pf_cech_count": (
+        c_int32,
+        [c_void_p, c_void_p, c_int64, ctypes.POINTER(PfCechGrid), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+         c_void_p, c_void_p, c_void_p],
+    ),
+    "pf_cech_write": (
+        c_int32,
+        [c_void_p, c_void_p, c_int64, ctypes.POINTER(PfCechGrid), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+         c_void_p],
+    ),
     "pf_sh_forward": (
         c_int32,
         [c_void_p, c_int32, c_void_p, c_int64, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_int64, c_void_p],

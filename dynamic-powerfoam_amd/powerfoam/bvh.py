@@ -1,0 +1,75 @@
+"""Čech-complex adjacency — the drop-in for ``powerfoam/bvh.py``'s ``AABBTree``.
+
+``AABBTree(device)``, ``.update(centers, radii)`` and
+``.build_cech_complex() -> (adjacency int32 (E,), offsets int32 (N+1,))``
+keep the reference's interface (bvh.py:72-123).  The Warp BVH is replaced by
+a uniform grid (cell size >= 2 max radius) built and queried by gfx950
+kernels (``csrc/cech.hip``); like the reference, the build reads the total
+edge count back to the host once to size the CSR (bvh.py:110).
+"""
+import ctypes
+import math
+
+import torch
+
+from . import _lib
+
+_MAX_GRID_CELLS = 1 << 25
+
+
+class AABBTree:
+    def __init__(self, device):
+        self.device = device
+        _lib.load()
+
+    def update(self, centers, radii):
+        if centers.device.type != "cuda":
+            raise ValueError("Only CUDA tensors are supported as inputs")
+        self.centers = centers.detach().to(torch.float32).contiguous()
+        self.radii = radii.detach().to(torch.float32).contiguous()
+        lib = _lib.load()
+        ws = torch.empty(8, dtype=torch.int32, device=self.centers.device)
+        bounds = torch.empty(8, dtype=torch.float32, device=self.centers.device)
+        _lib.check(lib.pf_cech_bounds(_lib.ptr(self.centers), _lib.ptr(self.radii), self.centers.shape[0],
+                                      _lib.ptr(ws), _lib.ptr(bounds), _lib.stream_ptr()), "pf_cech_bounds")
+        b = bounds.cpu().tolist()
+        lo, hi, rmax = b[0:3], b[3:6], b[6]
+        cell = max(2.0 * rmax * 1.001, 1e-6)
+        ext = [max(hi[k] - lo[k], 0.0) for k in range(3)]
+        dims = [int(math.floor(e / cell)) + 1 for e in ext]
+        while dims[0] * dims[1] * dims[2] > _MAX_GRID_CELLS:
+            cell *= 1.5
+            dims = [int(math.floor(e / cell)) + 1 for e in ext]
+        g = _lib.PfCechGrid()
+        for k in range(3):
+            g.origin[k] = lo[k]
+            g.dims[k] = dims[k]
+        g.cell = cell
+        self.grid = g
+
+    def build_cech_complex(self):
+        lib = _lib.load()
+        dev = self.centers.device
+        n = self.centers.shape[0]
+        g = self.grid
+        G = g.dims[0] * g.dims[1] * g.dims[2]
+        gcount = torch.zeros(G + 1, dtype=torch.int32, device=dev)
+        gfill = torch.zeros(G, dtype=torch.int32, device=dev)
+        key = torch.empty(n, dtype=torch.int32, device=dev)
+        gstart = torch.empty(G + 1, dtype=torch.int64, device=dev)
+        items = torch.empty(n, dtype=torch.int32, device=dev)
+        rcount = torch.zeros(n + 1, dtype=torch.int32, device=dev)
+        roff = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        st = _lib.stream_ptr()
+        _lib.check(lib.pf_cech_count(_lib.ptr(self.centers), _lib.ptr(self.radii), n, ctypes.byref(g),
+                                     _lib.ptr(gcount), _lib.ptr(gfill), _lib.ptr(key), _lib.ptr(gstart),
+                                     _lib.ptr(items), _lib.ptr(rcount), _lib.ptr(roff), st), "pf_cech_count")
+        total = int(roff[-1].item())
+        if total >= 2**31:
+            raise ValueError(f"Čech complex has {total} edges; int32 offsets overflow")
+        adjacent = torch.zeros(max(total, 1), dtype=torch.int32, device=dev)
+        offsets = torch.empty(n + 1, dtype=torch.int32, device=dev)
+        _lib.check(lib.pf_cech_write(_lib.ptr(self.centers), _lib.ptr(self.radii), n, ctypes.byref(g),
+                                     _lib.ptr(gstart), _lib.ptr(items), _lib.ptr(roff), _lib.ptr(adjacent),
+                                     _lib.ptr(offsets), st), "pf_cech_write")
+        return adjacent[:total], offsets

@@ -158,6 +158,156 @@ int pf_start_cells(const float *points, const float *radii, int64_t num_cells,
                    pf_camera *cams, int32_t num_cams, void *workspace,
                    void *stream);
 
+/* ---- tile rasterizer (Rasterizer) ----------------------------------------- */
+
+/* Outputs of Rasterizer._forward (rasterize.py:1639-1679, 1850-1860);
+ * (H,W[,3|Q]) row-major, contrib/point_err (N,) accumulated (+=).         */
+typedef struct pf_raster_out {
+    float *color;            /* (H,W,3)                                     */
+    float *log_t;            /* (H,W)                                       */
+    float *normal_distance;  /* (H,W)                                       */
+    float *normal;           /* (H,W,3)                                     */
+    float *quantile_depths;  /* (H,W,Q), caller fills with -1; may be NULL  */
+    float *err;              /* (H,W) surface objective only; may be NULL   */
+    float *contrib;          /* (N,)                                        */
+    float *point_err;        /* (N,) when return_point_err; may be NULL     */
+} pf_raster_out;
+
+/* Incoming gradients of RasterGradFn.backward (rasterize.py:1865-1877),
+ * after grad_log_t = -grad_opacity * exp(log_t) (:1909).  color required. */
+typedef struct pf_raster_grad_in {
+    const float *color;            /* (H,W,3)                               */
+    const float *log_t;            /* (H,W) or NULL                         */
+    const float *normal_distance;  /* (H,W) or NULL                         */
+    const float *normal;           /* (H,W,3) or NULL                       */
+    const float *quantile_depths;  /* (H,W,Q) or NULL                       */
+    const float *contrib;          /* (N,) or NULL                          */
+} pf_raster_grad_in;
+
+/* VisOptions (rasterize.py:15-20).                                         */
+typedef struct pf_vis_options {
+    float transmittance_threshold;
+    int32_t max_intersections;
+    float depth_quantile;
+    float bkgd_color[3];
+} pf_vis_options;
+
+/* Rasterizer.visualize outputs (rasterize.py:2178-2202).                  */
+typedef struct pf_vis_out {
+    float *color;          /* (H,W,3) */
+    float *depth;          /* (H,W)   */
+    float *normal;         /* (H,W,3) */
+    float *alpha;          /* (H,W)   */
+    int32_t *intersections;/* (H,W)   */
+} pf_vis_out;
+
+/* Raster faces (prefetch_adjacency_kernel, rasterize.py:580-606): for every
+ * CSR edge i->j, faces[e] = (p_j - p_i, r_j) in face_dtype (PF_F16 as the
+ * reference).  spheres (N,4) f32.                                          */
+int pf_raster_faces(const float *spheres, int64_t num_cells,
+                    const int32_t *adjacency, const int32_t *adjacency_offsets,
+                    void *faces, int32_t face_dtype, void *stream);
+
+/* Binning pass 1 (count_visible_pinhole_kernel, rasterize.py:140-200):
+ * tile_counts[1 + tile] += number of cells whose projected OBB passes the
+ * tile SAT.  tile_counts has num_tiles+1 entries and is zeroed by the
+ * caller.  cam: one device pf_camera.                                       */
+int pf_raster_bin_count(const float *spheres, int64_t num_cells,
+                        const pf_camera *cam, int32_t *tile_counts,
+                        void *stream);
+
+/* Inclusive scan of n int32 counts into int64 offsets (the reference's
+ * torch.cumsum, rasterize.py:1717).                                        */
+int pf_raster_scan(const int32_t *counts, int64_t n, int64_t *offsets,
+                   void *stream);
+
+/* Binning pass 2 (write_visible_pinhole_kernel, rasterize.py:293-363) and
+ * the per-tile ordering (torch.argsort of the keys, :1771): each tile's
+ * cells sorted by power distance from the eye (bit pattern of the positive
+ * f32), ties broken by cell index.  write_counter (num_tiles) zeroed by the
+ * caller; keys and scratch: num_isect u64 each; tile_prims (num_isect) out;
+ * visible_mask (N bytes, caller-zeroed) or NULL gets 1 for listed cells.  */
+int pf_raster_bin_write(const float *spheres, int64_t num_cells,
+                        const pf_camera *cam, int32_t num_tiles,
+                        const int64_t *tile_offsets, int32_t *write_counter,
+                        uint64_t *keys, uint64_t *scratch, int32_t *tile_prims,
+                        uint8_t *visible_mask, int64_t num_isect, void *stream);
+
+/* forward_kernel (rasterize.py:892-1146): volume objective, f32 attributes,
+ * scene.faces = raster faces.  tile_early_stop (num_tiles) out: the list
+ * position where the tile's maximum transmittance fell below the
+ * threshold (or the list length).  depth_quantiles (H,W,Q) or NULL;
+ * ray_gt (H,W,3) needed for point_err / the surface objective.            */
+int pf_raster_forward(const pf_scene *scene, const pf_camera *cam,
+                      int32_t width, int32_t height,
+                      const int64_t *tile_offsets, const int32_t *tile_prims,
+                      int32_t *tile_early_stop, float transmittance_threshold,
+                      const float *depth_quantiles, int32_t num_quantiles,
+                      const float *ray_gt, int32_t return_point_err,
+                      int32_t surface_objective, const pf_raster_out *out,
+                      void *stream);
+
+/* backward_kernel (rasterize.py:1150-1516): reverse replay from each
+ * tile's early-stop position and the saved per-pixel log_t; gradients
+ * accumulated into grads (texel_rgb_cam_stride ignored).  K <= 8.          */
+int pf_raster_backward(const pf_scene *scene, const pf_camera *cam,
+                       int32_t width, int32_t height,
+                       const int64_t *tile_offsets, const int32_t *tile_prims,
+                       const int32_t *tile_early_stop,
+                       const float *depth_quantiles, int32_t num_quantiles,
+                       const float *log_t, const pf_raster_grad_in *gin,
+                       const pf_grads *grads, void *stream);
+
+/* benchmark_kernel (rasterize.py:614-734): colour only, per-pixel early
+ * stop, scene.faces = walk faces (dp, dpm) in f16, attributes f32 or f16.  */
+int pf_raster_benchmark(const pf_scene *scene, const pf_camera *cam,
+                        int32_t width, int32_t height,
+                        const int64_t *tile_offsets, const int32_t *tile_prims,
+                        float transmittance_threshold, float *color,
+                        void *stream);
+
+/* visualization_kernel (rasterize.py:738-888): scene.faces = raster faces
+ * in f16; attributes f32 or f16.                                           */
+int pf_raster_visualize(const pf_scene *scene, const pf_camera *cam,
+                        int32_t width, int32_t height,
+                        const int64_t *tile_offsets, const int32_t *tile_prims,
+                        const pf_vis_options *options, const pf_vis_out *out,
+                        void *stream);
+
+/* ---- Čech complex (AABBTree) ------------------------------------------- */
+
+/* Uniform grid for the overlap search: cell >= 2 max radius.             */
+typedef struct pf_cech_grid {
+    float origin[3];
+    float cell;
+    int32_t dims[3];
+    int32_t reserved;
+} pf_cech_grid;
+
+/* Bounds of the recovered sphere centres and the largest radius
+ * (AABBTree.update, bvh.py:76-94): bounds[0..2] min, [3..5] max, [6] r_max
+ * (device floats).  workspace: 7 int32 (device).                           */
+int pf_cech_bounds(const float *centers, const float *radii, int64_t num_cells,
+                   int32_t *workspace, float *bounds, void *stream);
+
+/* count_adjacent (bvh.py:7-33) plus its cumsum (:109): buckets the spheres
+ * into the grid, counts each sphere's overlapping neighbours and scans the
+ * counts into row_offsets (N+1, int64).  grid_counts (G+1) and grid_fill
+ * (G) zeroed by the caller; cell_key, grid_items: N; grid_start: G+1;
+ * row_counts: N+1 with row_counts[0] = 0.                                  */
+int pf_cech_count(const float *centers, const float *radii, int64_t num_cells,
+                  const pf_cech_grid *grid, int32_t *grid_counts,
+                  int32_t *grid_fill, int32_t *cell_key, int64_t *grid_start,
+                  int32_t *grid_items, int32_t *row_counts,
+                  int64_t *row_offsets, void *stream);
+
+/* write_adjacent (bvh.py:36-69): fills each CSR row (sorted ascending) and
+ * the int32 offsets (N+1).  adjacency holds row_offsets[N] entries.        */
+int pf_cech_write(const float *centers, const float *radii, int64_t num_cells,
+                  const pf_cech_grid *grid, const int64_t *grid_start,
+                  const int32_t *grid_items, const int64_t *row_offsets,
+                  int32_t *adjacency, int32_t *adjacency_offsets, void *stream);
+
 /* ---- colour pre-pass ---------------------------------------------------- */
 
 /* Spherical-harmonics colour per texel site and camera (degree 0..3,

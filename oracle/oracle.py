@@ -214,3 +214,140 @@ def walk_loss_f64(params, adjacency, offsets, cams, target, thr=1e-3, w_alpha=0.
                                  _p(P["texel_rgb"]), _p(P["texel_height"]), K, N, _p(adjacency), _p(offsets),
                                  adjacency.size, _p(cams), cams.shape[0], thr, max_steps, _p(r),
                                  0 if r is None else r.size, _p(target), w_alpha, nthreads or num_threads())
+
+
+# ------------------------------------------------------------ rasterizer --
+class RScene(ctypes.Structure):
+    _fields_ = [("spheres", c_void_p), ("nsigmas", c_void_p), ("sites", c_void_p), ("rgb", c_void_p),
+                ("height", c_void_p), ("adj", c_void_p), ("off", c_void_p), ("faces", c_void_p), ("N", c_int64),
+                ("K", c_int32), ("face_f16", c_int32)]
+
+
+def _rlib():
+    lib = load()
+    if not getattr(lib, "_raster_sigs", False):
+        lib.orc_raster_faces.argtypes = [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int32]
+        lib.orc_raster_bin.restype = c_int64
+        lib.orc_raster_bin.argtypes = [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64]
+        lib.orc_raster_forward.argtypes = ([c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_int32,
+                                            c_void_p, c_int32, c_int32] + [c_void_p] * 9 + [c_int32])
+        lib.orc_raster_backward.argtypes = ([c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32]
+                                            + [c_void_p] * 12 + [c_int32])
+        lib.orc_raster_plain.argtypes = ([c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int32, c_int32,
+                                          c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                          c_int32])
+        lib._raster_sigs = True
+    return lib
+
+
+class RasterScene:
+    def __init__(self, points, radii, density, normals, texel_sites, texel_rgb, texel_height, adjacency, offsets,
+                 faces):
+        self.spheres = _c(np.concatenate([points, radii[:, None]], 1), np.float32)
+        self.nsigmas = _c(np.concatenate([normals, density[:, None]], 1), np.float32)
+        self.sites, self.rgb = _c(texel_sites, np.float32), _c(texel_rgb, np.float32)
+        self.height = _c(texel_height, np.float32)
+        self.adj, self.off = _c(adjacency, np.int32), _c(offsets, np.int32)
+        self.faces = np.ascontiguousarray(faces)
+        f16 = self.faces.dtype == np.uint16
+        if not f16:
+            self.faces = _c(self.faces, np.float32)
+        s = RScene()
+        s.spheres, s.nsigmas, s.sites, s.rgb, s.height = (_p(self.spheres), _p(self.nsigmas), _p(self.sites),
+                                                          _p(self.rgb), _p(self.height))
+        s.adj, s.off, s.faces = _p(self.adj), _p(self.off), _p(self.faces)
+        s.N = self.spheres.shape[0]
+        s.K = self.sites.shape[1]
+        s.face_f16 = 1 if f16 else 0
+        self.c = s
+        self.N, self.K = s.N, s.K
+
+
+def raster_faces(spheres, adjacency, offsets, f16=True):
+    spheres = _c(spheres, np.float32)
+    adjacency, offsets = _c(adjacency, np.int32), _c(offsets, np.int32)
+    out = np.empty((adjacency.size, 4), np.uint16 if f16 else np.float32)
+    _rlib().orc_raster_faces(_p(spheres), spheres.shape[0], _p(adjacency), _p(offsets), _p(out), 1 if f16 else 0)
+    return out
+
+
+def raster_bin(spheres, cam_words):
+    lib = _rlib()
+    spheres = _c(spheres, np.float32)
+    cam = _c(cam_words, np.int32).reshape(12)
+    W, H = int(cam[10]), int(cam[11])
+    tiles = ((H + 7) // 8) * ((W + 7) // 8)
+    offsets = np.zeros(tiles + 1, np.int64)
+    n = lib.orc_raster_bin(_p(spheres), spheres.shape[0], _p(cam), _p(offsets), None, 0)
+    prims = np.zeros(max(n, 1), np.int32)
+    lib.orc_raster_bin(_p(spheres), spheres.shape[0], _p(cam), _p(offsets), _p(prims), n)
+    return offsets, prims[:n]
+
+
+def raster_forward(scene, cam_words, offsets, prims, thr=1e-3, depth_q=None, ray_gt=None, point_err=False,
+                   surface=False, nthreads=None):
+    lib = _rlib()
+    cam = _c(cam_words, np.int32).reshape(12)
+    W, H = int(cam[10]), int(cam[11])
+    tiles = offsets.size - 1
+    Q = 0 if depth_q is None else depth_q.shape[-1]
+    dq = None if depth_q is None else _c(depth_q, np.float32)
+    gt = None if ray_gt is None else _c(ray_gt, np.float32)
+    out = dict(color=np.zeros((H, W, 3), np.float32), log_t=np.zeros((H, W), np.float32),
+               ndist=np.zeros((H, W), np.float32), normal=np.zeros((H, W, 3), np.float32),
+               qdepth=None if dq is None else -np.ones(dq.shape, np.float32),
+               err=np.zeros((H, W), np.float32) if surface else None, contrib=np.zeros(scene.N, np.float32),
+               perr=np.zeros(scene.N, np.float32) if point_err else None, early=np.zeros(tiles, np.int32))
+    offsets, prims = _c(offsets, np.int64), _c(prims, np.int32)
+    lib.orc_raster_forward(ctypes.byref(scene.c), _p(cam), _p(offsets), _p(prims), thr, _p(dq), Q, _p(gt),
+                           int(point_err), int(surface), _p(out["color"]), _p(out["log_t"]), _p(out["ndist"]),
+                           _p(out["normal"]), _p(out["qdepth"]), _p(out["err"]), _p(out["contrib"]),
+                           _p(out["perr"]), _p(out["early"]), nthreads or num_threads())
+    return out
+
+
+def raster_backward(scene, cam_words, offsets, prims, early, log_t, g_color, g_log_t=None, g_ndist=None,
+                    g_normal=None, depth_q=None, g_qdepth=None, g_contrib=None, nthreads=None):
+    lib = _rlib()
+    cam = _c(cam_words, np.int32).reshape(12)
+    n, K = scene.N, scene.K
+    g = dict(spheres=np.zeros((n, 4), np.float32), nsigmas=np.zeros((n, 4), np.float32),
+             texel_sites=np.zeros((n, K, 3), np.float32), texel_rgb=np.zeros((n, K, 3), np.float32),
+             texel_height=np.zeros((n, K), np.float32))
+    cv = lambda a: None if a is None else _c(a, np.float32)
+    args = [cv(depth_q), cv(log_t), cv(g_color), cv(g_log_t), cv(g_ndist), cv(g_normal), cv(g_qdepth), cv(g_contrib)]
+    Q = 0 if depth_q is None else depth_q.shape[-1]
+    lib.orc_raster_backward(ctypes.byref(scene.c), _p(cam), _p(_c(offsets, np.int64)), _p(_c(prims, np.int32)),
+                            _p(_c(early, np.int32)), _p(args[0]), Q, *[_p(a) for a in args[1:]],
+                            _p(g["spheres"]), _p(g["nsigmas"]), _p(g["texel_sites"]), _p(g["texel_rgb"]),
+                            _p(g["texel_height"]), nthreads or num_threads())
+    return g
+
+
+def raster_plain(scene, cam_words, offsets, prims, thr=1e-3, vis=False, max_isect=1024, depth_qv=0.5,
+                 bkgd=(0.0, 0.0, 0.0), nthreads=None):
+    lib = _rlib()
+    cam = _c(cam_words, np.int32).reshape(12)
+    W, H = int(cam[10]), int(cam[11])
+    out = dict(color=np.zeros((H, W, 3), np.float32), depth=np.zeros((H, W), np.float32),
+               normal=np.zeros((H, W, 3), np.float32), alpha=np.zeros((H, W), np.float32),
+               isect=np.zeros((H, W), np.int32))
+    bk = np.asarray(bkgd, np.float32)
+    lib.orc_raster_plain(ctypes.byref(scene.c), _p(cam), _p(_c(offsets, np.int64)), _p(_c(prims, np.int32)), thr,
+                         int(vis), max_isect, depth_qv, _p(bk), _p(out["color"]), _p(out["depth"]), _p(out["normal"]),
+                         _p(out["alpha"]), _p(out["isect"]), nthreads or num_threads())
+    return out
+
+
+def cech(centers, radii):
+    """Brute-force Čech CSR (rows ascending): (adjacency int32, offsets int32)."""
+    lib = load()
+    lib.orc_cech.restype = c_int64
+    lib.orc_cech.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64]
+    centers, radii = _c(centers, np.float32), _c(radii, np.float32)
+    n = centers.shape[0]
+    off = np.zeros(n + 1, np.int32)
+    total = lib.orc_cech(_p(centers), _p(radii), n, _p(off), None, 0)
+    adj = np.zeros(max(total, 1), np.int32)
+    lib.orc_cech(_p(centers), _p(radii), n, _p(off), _p(adj), total)
+    return adj[:total], off
