@@ -1,0 +1,483 @@
+// pf_walk.h — the power-cell walk of one ray (forward, and backward by
+// replay) and its texel lookup, as device functions shared by the plain
+// walk kernels (walk.hip), the rasterizer's benchmark / visualize kernels
+// (raster.hip) and the serving kernel (serve.hip).
+//
+// Forward semantics follow RayTracer.benchmark_kernel
+// (powerfoam/raytrace.py:115-241) and plane_intersection_fwd_local
+// (raytrace.py:42-113).  A texel's colour comes from a colour source: stored
+// texel_rgb (the reference's input), or spherical harmonics evaluated at the
+// texel site's view direction (identical bits to the SH pre-pass).
+#pragma once
+#include "pf_device.h"
+#include "pf_hip.h"
+#include "pf_sh.h"
+
+namespace pf {
+
+constexpr int kNoCell = 0x7FFFFFFF;
+constexpr uint64_t kFnvOffset = 14695981039346656037ull;
+constexpr uint64_t kFnvPrime = 1099511628211ull;
+constexpr float kTemp = 10.0f;  // texel weight temperature (raytrace.py:39)
+
+struct alignas(8) h4 { _Float16 x, y, z, w; };
+
+template <typename F> struct FaceT;
+template <> struct FaceT<_Float16> {
+    __device__ static __forceinline__ void load(const void *p, int64_t e, f3 &n, float &w) {
+        const h4 v = reinterpret_cast<const h4 *>(p)[e];
+        n = mk((float)v.x, (float)v.y, (float)v.z);
+        w = (float)v.w;
+    }
+};
+template <> struct FaceT<float> {
+    __device__ static __forceinline__ void load(const void *p, int64_t e, f3 &n, float &w) {
+        const float4 v = reinterpret_cast<const float4 *>(p)[e];
+        n = mk(v.x, v.y, v.z);
+        w = v.w;
+    }
+};
+
+template <typename A> __device__ __forceinline__ f3 ld3(const A *p) {
+    return mk((float)p[0], (float)p[1], (float)p[2]);
+}
+template <typename A> __device__ __forceinline__ void ld4(const A *p, f3 &n, float &s) {
+    n = mk((float)p[0], (float)p[1], (float)p[2]);
+    s = (float)p[3];
+}
+
+__device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
+
+// ---------------------------------------------------------- colour sources --
+template <typename A> struct MemRGBCell {
+    const A *p;
+    __device__ __forceinline__ f3 operator()(int i, f3) const { return ld3(p + 3 * i); }
+};
+template <typename A> struct MemRGB {
+    const A *base;
+    __device__ __forceinline__ MemRGBCell<A> cell(int64_t cell, int K) const {
+        return MemRGBCell<A>{base + 3 * cell * K};
+    }
+};
+template <int DEG> struct SHRGBCell {
+    const float *k;
+    f3 eye;
+    __device__ __forceinline__ f3 operator()(int, f3 site) const {
+        float Y[16];
+        return sh_color(sh_value<DEG>(k, site, eye, Y));
+    }
+};
+template <int DEG> struct SHRGB {
+    const float *coeffs;
+    f3 eye;
+    __device__ __forceinline__ SHRGBCell<DEG> cell(int64_t cell, int) const {
+        return SHRGBCell<DEG>{coeffs + cell * (DEG + 1) * (DEG + 1) * 3, eye};
+    }
+};
+
+// ------------------------------------------------------------ texel lookup --
+// Two passes over the K texel sites: height at the base-plane query point,
+// then colour at the displaced plane's query point; weights use the walk's
+// r^-2 multiply (raytrace.py:61, :68).
+struct TexFwd {
+    float t0, dp0, height, wsum0, t_surf, dp, wsum;
+    f3 rgb;
+};
+
+template <typename A, int KT, typename RGBCell>
+__device__ __forceinline__ TexFwd texel_fwd(f3 o, f3 d, float t_near, f3 c, f3 n, float r, const A *sites,
+                                            const RGBCell &rgbs, const A *heights, int K) {
+    TexFwd out;
+    if constexpr (KT > 0) K = KT;
+    ray_plane(o, d, c, n, 0.0f, out.t0, out.dp0);
+    const float tq0 = out.dp0 >= 0.0f ? t_near : fmax_(t_near, out.t0);
+    const f3 q0 = ray_at(o, tq0, d);
+    const float inv_r2 = 1.0f / (r * r);
+    float hs = 0.0f, ws = 0.0f;
+#pragma unroll
+    for (int i = 0; i < (KT > 0 ? KT : 16); ++i) {
+        if (KT == 0 && i >= K) break;
+        const f3 s = ld3(sites + 3 * i);
+        const f3 dv = q0 - s;
+        const float dsq = dot(dv, dv) * inv_r2;
+        const float w = pf_expf(-kTemp * dsq);
+        hs = __builtin_fmaf(w, (float)heights[i], hs);
+        ws = ws + w;
+    }
+    ws = fmax_(ws, 1e-20f);
+    out.wsum0 = ws;
+    out.height = hs / ws;
+    ray_plane(o, d, c, n, out.height, out.t_surf, out.dp);
+    const float tq = out.dp >= 0.0f ? t_near : fmax_(t_near, out.t_surf);
+    const f3 q = ray_at(o, tq, d);
+    f3 acc = mk(0.0f, 0.0f, 0.0f);
+    float ws2 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < (KT > 0 ? KT : 16); ++i) {
+        if (KT == 0 && i >= K) break;
+        const f3 s = ld3(sites + 3 * i);
+        const f3 dv = q - s;
+        const float dsq = dot(dv, dv) * inv_r2;
+        const float w = pf_expf(-kTemp * dsq);
+        acc = macc(acc, w, rgbs(i, s));
+        ws2 = ws2 + w;
+    }
+    ws2 = fmax_(ws2, 1e-20f);
+    out.wsum = ws2;
+    out.rgb = divs(acc, ws2);
+    return out;
+}
+
+template <typename A, int KT>
+__device__ __forceinline__ TexFwd texel_fwd(f3 o, f3 d, float t_near, f3 c, f3 n, float r, const A *sites,
+                                            const A *rgbs, const A *heights, int K) {
+    return texel_fwd<A, KT>(o, d, t_near, c, n, r, sites, MemRGBCell<A>{rgbs}, heights, K);
+}
+
+// Texel backward: the semantics of plane_intersection_bwd
+// (powerfoam/texture.py:74-186) with the walk's weights (r^-2 multiply).
+// Returns d/d(plane origin), d/d(normal) and d/d(t_near before the plane).
+// exact == false reproduces the reference's two approximations: texels
+// whose weight is below 1e-3 of the weight sum are skipped
+// (texture.py:118-119, :158-159) and the radius gets no gradient through
+// the texel weights.  exact == true keeps every texel and adds that radius
+// term, d w_k / d r = 2 temp w_k |q - s_k|^2 / r^3, into dL_dr.
+template <typename A, int KT, typename RGBCell>
+__device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 n, float r, const A *sites,
+                                          const RGBCell &rgbs, const A *heights, int K, const TexFwd &fw,
+                                          f3 dL_drgb, float dL_dtsurf_in, float *g_sites, float *g_rgb,
+                                          float *g_height, f3 &dL_dc, f3 &dL_dn, float &dL_dtnear, bool exact,
+                                          float &dL_dr) {
+    if constexpr (KT > 0) K = KT;
+    dL_dc = mk(0, 0, 0);
+    dL_dn = mk(0, 0, 0);
+    float dL_dheight = 0.0f;
+    const float inv_r2 = 1.0f / (r * r);
+    const float r2 = r * r;
+
+    const float tq = fw.dp >= 0.0f ? t_near_in : fmax_(t_near_in, fw.t_surf);
+    const f3 q = ray_at(o, tq, d);
+    f3 dL_dq = mk(0, 0, 0);
+    const float gdp_in = dot(dL_drgb, fw.rgb);
+#pragma unroll
+    for (int i = 0; i < (KT > 0 ? KT : 16); ++i) {
+        if (KT == 0 && i >= K) break;
+        const f3 s = ld3(sites + 3 * i);
+        const f3 dv = q - s;
+        const float dsq = dot(dv, dv) * inv_r2;
+        const float w = pf_expf(-kTemp * dsq);
+        if (!exact && w / fw.wsum < 1e-3f) continue;
+        const f3 dw_ds = divs(((2.0f * kTemp) * w) * dv, r2);
+        const f3 grgb = divs(w * dL_drgb, fw.wsum);
+        if (grgb.x != 0.0f) atomic_add(g_rgb + 3 * i + 0, grgb.x);
+        if (grgb.y != 0.0f) atomic_add(g_rgb + 3 * i + 1, grgb.y);
+        if (grgb.z != 0.0f) atomic_add(g_rgb + 3 * i + 2, grgb.z);
+        const float gdp = dot(dL_drgb, rgbs(i, s));
+        const f3 gs = divs(dw_ds * (gdp - gdp_in), fw.wsum);
+        if (gs.x != 0.0f) atomic_add(g_sites + 3 * i + 0, gs.x);
+        if (gs.y != 0.0f) atomic_add(g_sites + 3 * i + 1, gs.y);
+        if (gs.z != 0.0f) atomic_add(g_sites + 3 * i + 2, gs.z);
+        dL_dq = dL_dq + (-1.0f * gs);
+        if (exact) dL_dr += ((gdp - gdp_in) / fw.wsum) * ((((2.0f * kTemp) * w) * dot(dv, dv)) / (r2 * r));
+    }
+    const float dL_dtq = dot(dL_dq, d);
+    dL_dtnear = (fw.dp >= 0.0f || t_near_in >= fw.t_surf) ? dL_dtq : 0.0f;
+    float dL_dts = (fw.dp < 0.0f && t_near_in < fw.t_surf) ? dL_dtq : 0.0f;
+    dL_dts += dL_dtsurf_in;
+    if (__builtin_fabsf(dL_dts) > 0.0f) {
+        f3 dtdp, dtdn;
+        float dtdh;
+        ray_plane_bwd(o, d, c, n, fw.height, dtdp, dtdn, dtdh);
+        dL_dc = dL_dc + dtdp * dL_dts;
+        dL_dn = dL_dn + dtdn * dL_dts;
+        dL_dheight += dtdh * dL_dts;
+    }
+
+    const float tq0 = fw.dp0 >= 0.0f ? t_near_in : fmax_(t_near_in, fw.t0);
+    const f3 q0 = ray_at(o, tq0, d);
+    f3 dL_dq0 = mk(0, 0, 0);
+    const float hdp_in = dL_dheight * fw.height;
+    if (dL_dheight != 0.0f) {
+#pragma unroll
+        for (int i = 0; i < (KT > 0 ? KT : 16); ++i) {
+            if (KT == 0 && i >= K) break;
+            const f3 s = ld3(sites + 3 * i);
+            const f3 dv = q0 - s;
+            const float dsq = dot(dv, dv) * inv_r2;
+            const float w = pf_expf(-kTemp * dsq);
+            if (!exact && w / fw.wsum0 < 1e-3f) continue;
+            const f3 dw_ds = divs(((2.0f * kTemp) * w) * dv, r2);
+            const float gh = (w * dL_dheight) / fw.wsum0;
+            if (gh != 0.0f) atomic_add(g_height + i, gh);
+            const float hdp = dL_dheight * (float)heights[i];
+            const f3 gs = divs(dw_ds * (hdp - hdp_in), fw.wsum0);
+            if (gs.x != 0.0f) atomic_add(g_sites + 3 * i + 0, gs.x);
+            if (gs.y != 0.0f) atomic_add(g_sites + 3 * i + 1, gs.y);
+            if (gs.z != 0.0f) atomic_add(g_sites + 3 * i + 2, gs.z);
+            dL_dq0 = dL_dq0 + (-1.0f * gs);
+            if (exact) dL_dr += ((hdp - hdp_in) / fw.wsum0) * ((((2.0f * kTemp) * w) * dot(dv, dv)) / (r2 * r));
+        }
+    }
+    const float dL_dtq0 = dot(dL_dq0, d);
+    dL_dtnear += (fw.dp >= 0.0f || t_near_in >= fw.t0) ? dL_dtq0 : 0.0f;
+    const float dL_dts0 = (fw.dp < 0.0f && t_near_in < fw.t0) ? dL_dtq0 : 0.0f;
+    if (__builtin_fabsf(dL_dts0) > 0.0f) {
+        f3 dtdp, dtdn;
+        float dtdh;
+        ray_plane_bwd(o, d, c, n, 0.0f, dtdp, dtdn, dtdh);
+        dL_dc = dL_dc + dtdp * dL_dts0;
+        dL_dn = dL_dn + dtdn * dL_dts0;
+    }
+}
+
+// --------------------------------------------------------------- the walk --
+// Algorithmic bytes (what a visit, a face or a shaded segment must read at
+// least once): sphere 16 B + CSR row bounds 8 B + [n, sigma]; per face the
+// neighbour index and the face; per shaded segment K texels of
+// (site, colour, height).
+template <typename A, typename F> struct WalkBytes {
+    static constexpr uint64_t visit = 16 + 8 + 4 * sizeof(A);
+    static constexpr uint64_t face = 4 + 4 * sizeof(F);
+    static constexpr uint64_t texel = 7 * sizeof(A);
+};
+
+template <typename A> struct SceneDev {
+    const float4 *spheres;
+    const A *nsig, *sites, *height;
+    const int *adj, *off;
+    const void *faces;
+    int N, K;
+};
+
+template <typename A> __device__ __forceinline__ SceneDev<A> scene_dev(const pf_scene &s) {
+    return SceneDev<A>{reinterpret_cast<const float4 *>(s.spheres), reinterpret_cast<const A *>(s.nsigmas),
+                       reinterpret_cast<const A *>(s.texel_sites), reinterpret_cast<const A *>(s.texel_height),
+                       s.adjacency, s.adjacency_offsets, s.faces, (int)s.num_cells, s.num_texel_sites};
+}
+
+struct RayFwd {
+    f3 rgb;
+    float log_t, depth;
+    int visits, status;
+    uint64_t hash, bytes;
+};
+
+// raytrace.py:157-241; depth as rasterize.py:871-877.
+template <typename A, typename F, int KT, typename RGB>
+__device__ __forceinline__ RayFwd walk_ray_fwd(const SceneDev<A> &s, const RGB &colors, f3 o, f3 d, int start,
+                                               float thr, float depth_q, int max_steps, int *seq, int seq_cap) {
+    constexpr uint64_t kVisitBytes = WalkBytes<A, F>::visit, kFaceBytes = WalkBytes<A, F>::face,
+                       kTexelBytes = WalkBytes<A, F>::texel;
+    const int K = KT > 0 ? KT : s.K;
+    RayFwd out{mk(0, 0, 0), 0.0f, 0.0f, 0, 0, kFnvOffset, 0};
+    bool depth_found = false;
+    int prim = start;
+    while (true) {
+        const float trans = pf_expf(out.log_t);
+        if (trans < thr || prim == kNoCell) break;
+        if ((unsigned)prim >= (unsigned)s.N || out.visits >= max_steps) { out.status |= PF_STATUS_STEP_CAP; break; }
+        if (seq && out.visits < seq_cap) seq[out.visits] = prim;
+        ++out.visits;
+        out.hash = (out.hash ^ (uint64_t)(uint32_t)prim) * kFnvPrime;
+
+        const float4 sp = s.spheres[prim];
+        const f3 c = mk(sp.x, sp.y, sp.z);
+        const float r = sp.w;
+        SphereHit sh = ray_sphere(o, d, c, r);
+        if (length(c - o) < 4.0f * r) sh.hit = false;
+        float tn = sh.tn, tf = sh.tf;
+
+        const int e0 = s.off[prim];
+        const int e1 = s.off[prim + 1];
+        out.bytes += kVisitBytes + (uint64_t)(e1 - e0) * kFaceBytes;
+        int next = kNoCell;
+        float pt_far = 1e10f;
+        for (int e = e0; e < e1; ++e) {
+            const int nb = s.adj[e];
+            f3 fn;
+            float fw;
+            FaceT<F>::load(s.faces, e, fn, fw);
+            float t, dp;
+            ray_pface_diff(o, d, fn, fw, t, dp);
+            if (dp >= 0.0f && t < pt_far) { next = nb; pt_far = t; }
+            tf = dp >= 0.0f ? fmin_(t, tf) : tf;
+            tn = dp < 0.0f ? fmax_(t, tn) : tn;
+        }
+        f3 nrm;
+        float sigma;
+        ld4(s.nsig + 4 * (int64_t)prim, nrm, sigma);
+        if (!sh.hit || tn > tf || sigma < 1e-3f) { prim = next; continue; }
+
+        const int64_t tbase = (int64_t)prim * K;
+        out.bytes += (uint64_t)K * kTexelBytes;
+        const TexFwd tx = texel_fwd<A, KT>(o, d, tn, c, nrm, r, s.sites + 3 * tbase, colors.cell(prim, K),
+                                          s.height + tbase, K);
+        tf = tx.dp >= 0.0f ? fmin_(tx.t_surf, tf) : tf;
+        tn = tx.dp < 0.0f ? fmax_(tx.t_surf, tn) : tn;
+        prim = next;
+        const float dt = tf - tn;
+        if (dt > 0.0f) {
+            const float dlt = (-sigma) * dt;
+            const float alpha = 1.0f - pf_expf(dlt);
+            out.rgb.x = __builtin_fmaf(tx.rgb.x * alpha, trans, out.rgb.x);
+            out.rgb.y = __builtin_fmaf(tx.rgb.y * alpha, trans, out.rgb.y);
+            out.rgb.z = __builtin_fmaf(tx.rgb.z * alpha, trans, out.rgb.z);
+            out.log_t += dlt;
+            if (!depth_found) {
+                const float next_trans = pf_expf(out.log_t);
+                if (next_trans < depth_q) {
+                    out.depth = tn + pf_logf(trans / depth_q) / sigma;
+                    depth_found = true;
+                }
+            }
+        }
+    }
+    return out;
+}
+
+struct GradDev {
+    float *spheres, *nsigmas, *sites, *rgb, *height;
+};
+
+// Backward by checkpoint and replay.  The forward's per-ray colour C and
+// log transmittance are the checkpoint.  The ray replays its walk in
+// forward order with the forward's arithmetic (so the cell sequence and
+// every segment are bit-identical), and at each shaded segment s uses
+//   dL/d(delta_s) = g.(C - C_{<=s}) + g_logT - (g.c_s) T_s exp(delta_s)
+// which equals the reverse-order accumulation of the rasterizer backward
+// (rasterize.py:1371-1426).  The per-segment routing of dL/dt to the
+// sphere, a power face, or the texel plane mirrors rasterize.py:1296-1516.
+// Returns the status bits; adds the ray's algorithmic bytes to bytes.
+template <typename A, typename F, int KT, typename RGB>
+__device__ __forceinline__ int walk_ray_bwd(const SceneDev<A> &s, const RGB &colors, f3 o, f3 d, int start, float thr,
+                                            int max_steps, f3 Cfin, f3 g, float gL, const GradDev &G, bool exact,
+                                            uint64_t &bytes) {
+    constexpr uint64_t kVisitBytes = WalkBytes<A, F>::visit, kFaceBytes = WalkBytes<A, F>::face,
+                       kTexelBytes = WalkBytes<A, F>::texel;
+    const int K = KT > 0 ? KT : s.K;
+    f3 rgb = mk(0, 0, 0);
+    float log_t = 0.0f;
+    int prim = start;
+    int visits = 0;
+    while (true) {
+        const float trans = pf_expf(log_t);
+        if (trans < thr || prim == kNoCell) break;
+        if ((unsigned)prim >= (unsigned)s.N || visits >= max_steps) return PF_STATUS_STEP_CAP;
+        ++visits;
+
+        const float4 sp = s.spheres[prim];
+        const f3 c = mk(sp.x, sp.y, sp.z);
+        const float r = sp.w;
+        SphereHit sh = ray_sphere(o, d, c, r);
+        if (length(c - o) < 4.0f * r) sh.hit = false;
+        float tn = sh.tn, tf = sh.tf;
+        int tn_id = tn < 1e-8f ? -2 : -1;
+        int tf_id = tf < 1e-8f ? -2 : -1;
+
+        const int e0 = s.off[prim];
+        const int e1 = s.off[prim + 1];
+        bytes += kVisitBytes + (uint64_t)(e1 - e0) * kFaceBytes;
+        const int n_adj = e1 - e0;
+        int next = kNoCell;
+        float pt_far = 1e10f;
+        for (int e = e0; e < e1; ++e) {
+            const int nb = s.adj[e];
+            f3 fn;
+            float fw;
+            FaceT<F>::load(s.faces, e, fn, fw);
+            float t, dp;
+            ray_pface_diff(o, d, fn, fw, t, dp);
+            if (dp >= 0.0f && t < pt_far) { next = nb; pt_far = t; }
+            if (dp >= 0.0f && t < tf) tf_id = e - e0;
+            tf = dp >= 0.0f ? fmin_(t, tf) : tf;
+            if (dp < 0.0f && t > tn) tn_id = e - e0;
+            tn = dp < 0.0f ? fmax_(t, tn) : tn;
+        }
+        f3 nrm;
+        float sigma;
+        ld4(s.nsig + 4 * (int64_t)prim, nrm, sigma);
+        if (!sh.hit || tn > tf || sigma < 1e-3f) { prim = next; continue; }
+
+        const int64_t tbase = (int64_t)prim * K;
+        bytes += (uint64_t)K * kTexelBytes;
+        const float tn0 = tn;
+        const int tn0_id = tn_id, tf0_id = tf_id;
+        const auto rgbs = colors.cell(prim, K);
+        const TexFwd tx = texel_fwd<A, KT>(o, d, tn, c, nrm, r, s.sites + 3 * tbase, rgbs, s.height + tbase, K);
+        if (tx.dp >= 0.0f && tx.t_surf < tf) tf_id = n_adj;
+        tf = tx.dp >= 0.0f ? fmin_(tx.t_surf, tf) : tf;
+        if (tx.dp < 0.0f && tx.t_surf > tn) tn_id = n_adj;
+        tn = tx.dp < 0.0f ? fmax_(tx.t_surf, tn) : tn;
+        const int cell = prim;
+        prim = next;
+        const float dt = tf - tn;
+        if (!(dt > 0.0f)) continue;
+
+        const float dlt = (-sigma) * dt;
+        const float e_dlt = pf_expf(dlt);
+        const float alpha = 1.0f - e_dlt;
+        rgb.x = __builtin_fmaf(tx.rgb.x * alpha, trans, rgb.x);
+        rgb.y = __builtin_fmaf(tx.rgb.y * alpha, trans, rgb.y);
+        rgb.z = __builtin_fmaf(tx.rgb.z * alpha, trans, rgb.z);
+        log_t += dlt;
+
+        bytes += 2ull * (32ull + (uint64_t)K * 28ull);  // read-modify-write of the cell's gradients
+        const float dL_dLnext = dot(g, Cfin - rgb) + gL;
+        const float dL_dalpha = dot(g, tx.rgb) * trans;
+        const f3 dL_dcolor = (g * trans) * alpha;
+        const float dL_ddelta = dL_dLnext + (-dL_dalpha * e_dlt);
+        const float dL_dsigma = dL_ddelta * (-dt);
+        const float dL_dtn = dL_ddelta * sigma;
+        const float dL_dtf = dL_ddelta * (-sigma);
+
+        float dL_dts = 0.0f;
+        if (tn_id == n_adj) dL_dts = dL_dtn;
+        else if (tf_id == n_adj) dL_dts = dL_dtf;
+
+        f3 dL_dc, dL_dn;
+        float dL_dtn0, dL_dr = 0.0f;
+        texel_bwd<A, KT>(o, d, tn0, c, nrm, r, s.sites + 3 * tbase, rgbs, s.height + tbase, K, tx, dL_dcolor,
+                         dL_dts, G.sites + 3 * tbase, G.rgb + 3 * tbase, G.height + tbase, dL_dc, dL_dn, dL_dtn0,
+                         exact, dL_dr);
+        if (tn_id != n_adj) dL_dtn0 += dL_dtn;
+        for (int side = 0; side < 2; ++side) {
+            const int tid = side == 0 ? tn0_id : tf0_id;
+            const float dLdt = side == 0 ? dL_dtn0 : (tf_id != n_adj ? dL_dtf : 0.0f);
+            if (tid == -1) {
+                const SphereGrad sg = ray_sphere_bwd(o, d, c, r);
+                if (side == 0) {
+                    dL_dc = dL_dc + sg.dn_dc * dLdt;
+                    dL_dr += dLdt * sg.dn_dr;
+                } else {
+                    dL_dc = dL_dc + sg.df_dc * dLdt;
+                    dL_dr += dLdt * sg.df_dr;
+                }
+            } else if (tid >= 0 && tid < n_adj) {
+                const int adjc = s.adj[e0 + tid];
+                const float4 asp = s.spheres[adjc];
+                const FaceGrad fg = ray_pface_bwd(o, d, c, r, mk(asp.x, asp.y, asp.z), asp.w);
+                dL_dc = dL_dc + fg.dc * dLdt;
+                dL_dr += dLdt * fg.dr;
+                float *ga = G.spheres + 4 * (int64_t)adjc;
+                atomic_add(ga + 0, dLdt * fg.dcj.x);
+                atomic_add(ga + 1, dLdt * fg.dcj.y);
+                atomic_add(ga + 2, dLdt * fg.dcj.z);
+                atomic_add(ga + 3, dLdt * fg.drj);
+                bytes += 32;  // neighbour sphere gradient read-modify-write
+            }
+        }
+        float *gs = G.spheres + 4 * (int64_t)cell;
+        atomic_add(gs + 0, dL_dc.x);
+        atomic_add(gs + 1, dL_dc.y);
+        atomic_add(gs + 2, dL_dc.z);
+        atomic_add(gs + 3, dL_dr);
+        float *gn = G.nsigmas + 4 * (int64_t)cell;
+        if (dL_dn.x != 0.0f) atomic_add(gn + 0, dL_dn.x);
+        if (dL_dn.y != 0.0f) atomic_add(gn + 1, dL_dn.y);
+        if (dL_dn.z != 0.0f) atomic_add(gn + 2, dL_dn.z);
+        atomic_add(gn + 3, dL_dsigma);
+    }
+    return 0;
+}
+
+}  // namespace pf

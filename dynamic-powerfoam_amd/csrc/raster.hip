@@ -13,7 +13,7 @@
 #include "pf_device.h"
 #include "pf_hip.h"
 #include "pf_launch.h"
-#include "pf_texel.h"
+#include "pf_walk.h"
 
 namespace pf {
 

@@ -8,40 +8,9 @@
 // powerfoam/sh_utils.py:4-74 (the standard real-SH ordering).
 #include "pf_device.h"
 #include "pf_hip.h"
+#include "pf_sh.h"
 
 namespace pf {
-
-constexpr float SH_C0 = 0.28209479177387814f;
-constexpr float SH_C1 = 0.4886025119029199f;
-constexpr float SH_C2_0 = 1.0925484305920792f, SH_C2_1 = -1.0925484305920792f, SH_C2_2 = 0.31539156525252005f,
-                SH_C2_3 = -1.0925484305920792f, SH_C2_4 = 0.5462742152960396f;
-constexpr float SH_C3_0 = -0.5900435899266435f, SH_C3_1 = 2.890611442640554f, SH_C3_2 = -0.4570457994644658f,
-                SH_C3_3 = 0.3731763325901154f, SH_C3_4 = -0.4570457994644658f, SH_C3_5 = 1.445305721320277f,
-                SH_C3_6 = -0.5900435899266435f;
-
-__device__ __forceinline__ void sh_basis(f3 dir, int degree, float *Y) {
-    const float x = dir.x, y = dir.y, z = dir.z;
-    Y[0] = SH_C0;
-    if (degree < 1) return;
-    Y[1] = -SH_C1 * y;
-    Y[2] = SH_C1 * z;
-    Y[3] = -SH_C1 * x;
-    if (degree < 2) return;
-    const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
-    Y[4] = SH_C2_0 * xy;
-    Y[5] = SH_C2_1 * yz;
-    Y[6] = SH_C2_2 * (2.0f * zz - xx - yy);
-    Y[7] = SH_C2_3 * xz;
-    Y[8] = SH_C2_4 * (xx - yy);
-    if (degree < 3) return;
-    Y[9] = SH_C3_0 * y * (3.0f * xx - yy);
-    Y[10] = SH_C3_1 * xy * z;
-    Y[11] = SH_C3_2 * y * (4.0f * zz - xx - yy);
-    Y[12] = SH_C3_3 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
-    Y[13] = SH_C3_4 * x * (4.0f * zz - xx - yy);
-    Y[14] = SH_C3_5 * z * (xx - yy);
-    Y[15] = SH_C3_6 * x * (xx - 3.0f * yy);
-}
 
 // One thread per (cell, texel): coefficients (N, (D+1)^2, 3) f32, sites
 // (N, K, 3) f32 -> rgb (N, K, 3) in the attribute dtype.
@@ -54,22 +23,13 @@ __global__ __launch_bounds__(256) void sh_fwd_kernel(const float *coeffs, const 
     constexpr int NC = (DEG + 1) * (DEG + 1);
     const f3 eye = mk(cams[12 * b], cams[12 * b + 1], cams[12 * b + 2]);
     const f3 s = mk(sites[3 * t], sites[3 * t + 1], sites[3 * t + 2]);
-    f3 dir = s - eye;
-    dir = divs(dir, length(dir));
     float Y[16];
-    sh_basis(dir, DEG, Y);
-    const float *k = coeffs + (t / K) * NC * 3;
-    float r = 0.0f, g = 0.0f, bl = 0.0f;
-#pragma unroll
-    for (int l = 0; l < NC; ++l) {
-        r = __builtin_fmaf(Y[l], k[3 * l + 0], r);
-        g = __builtin_fmaf(Y[l], k[3 * l + 1], g);
-        bl = __builtin_fmaf(Y[l], k[3 * l + 2], bl);
-    }
+    const f3 v = sh_value<DEG>(coeffs + (t / K) * NC * 3, s, eye, Y);
+    const f3 c = sh_color(v);
     A *o = rgb + (int64_t)b * rgb_cam_stride + 3 * t;
-    o[0] = (A)fmax_(r + 0.5f, 0.0f);
-    o[1] = (A)fmax_(g + 0.5f, 0.0f);
-    o[2] = (A)fmax_(bl + 0.5f, 0.0f);
+    o[0] = (A)c.x;
+    o[1] = (A)c.y;
+    o[2] = (A)c.z;
 }
 
 // One thread per cell: sums over its K texels and all B cameras, writes
@@ -94,17 +54,9 @@ __global__ __launch_bounds__(256) void sh_bwd_kernel(const float *coeffs, const 
             const float g0 = gr[0], g1 = gr[1], g2 = gr[2];
             if (g0 == 0.0f && g1 == 0.0f && g2 == 0.0f) continue;
             const f3 s = mk(sites[3 * t], sites[3 * t + 1], sites[3 * t + 2]);
-            f3 dir = s - eye;
-            dir = divs(dir, length(dir));
             float Y[16];
-            sh_basis(dir, DEG, Y);
-            float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
-#pragma unroll
-            for (int l = 0; l < NC; ++l) {
-                v0 = __builtin_fmaf(Y[l], k[3 * l + 0], v0);
-                v1 = __builtin_fmaf(Y[l], k[3 * l + 1], v1);
-                v2 = __builtin_fmaf(Y[l], k[3 * l + 2], v2);
-            }
+            const f3 v = sh_value<DEG>(k, s, eye, Y);
+            const float v0 = v.x, v1 = v.y, v2 = v.z;
             const float m0 = (v0 + 0.5f) > 0.0f ? g0 : 0.0f;
             const float m1 = (v1 + 0.5f) > 0.0f ? g1 : 0.0f;
             const float m2 = (v2 + 0.5f) > 0.0f ? g2 : 0.0f;

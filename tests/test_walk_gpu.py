@@ -206,7 +206,7 @@ def test_walk_gpu_gradients_match_fp64_finite_differences(scene_textured):
     gc = 2.0 * (fwd["color"] - torch.from_numpy(target).to(DEV))
     gl = -w_alpha * torch.exp(fwd["log_t"])
     g = rt.launch_walk_backward(gs, cams_dev, 24, 24, fwd["color"], fwd["log_t"], gc, gl, exact_texel=True)
-    g = {k: v.cpu().numpy() for k, v in g.items()}
+    g = {k: v.cpu().numpy() for k, v in g.items() if v is not None}
     P = dict(points=sc["points"].astype(np.float64), radii=sc["radii"].astype(np.float64),
              nsigmas=np.concatenate([sc["normals"], sc["density"][:, None]], 1).astype(np.float64),
              texel_sites=sc["texel_sites"].astype(np.float64), texel_rgb=sc["texel_rgb"].astype(np.float64),
