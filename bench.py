@@ -39,6 +39,8 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work (0 = skip)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--serve-requests", type=int, default=160, help="requests in the serving trace (0 = skip)")
+    ap.add_argument("--serve-load", type=float, default=0.6, help="offered load as a fraction of plain capacity")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -148,6 +150,10 @@ def main():
         f"({fwd_bytes / fwd_ms / 1e6:.0f} GB/s alg), bwd {bwd_ms:.2f} ms ({bwd_bytes / bwd_ms / 1e6:.0f} GB/s alg); "
         f"visits/ray mean {visits.mean().item():.1f} max {int(visits.max().item())}; status {status}")
 
+    serving = None
+    if args.serve_requests > 0:
+        serving = serve_bench(scene, coeffs, pts, rad, sc, args)
+
     cpu = None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(sc, adj.numpy(), off.numpy(), faces, texel_rgb, cams_dev, f, gc, args.cpu_seconds)
@@ -188,8 +194,86 @@ def main():
             "traffic": None,
         },
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
+
+
+def serve_bench(scene, coeffs, pts, rad, sc, args):
+    """Serving form on the resident 1M-cell scene: a Poisson stream of
+    forward requests, resolutions uniform over {256^2, 512^2, 1024^2,
+    1920x1080}, cameras on the radius-3 sphere.  The same trace is served by
+    the continuous-batching server and by the plain per-request path (SH
+    pre-pass + walk, one request at a time); the arrival rate is set to
+    serve_load x the plain path's measured capacity."""
+    import copy
+
+    import numpy as np
+    import torch
+
+    from powerfoam import workloads
+    from powerfoam.serving import PlainRenderer, RenderRequest, RenderServer, ResidentScene, latency_stats
+
+    res = [(256, 256), (512, 512), (1024, 1024), (1920, 1080)]
+    rs = ResidentScene(pts, rad, scene.nsigmas[:, 3], scene.nsigmas[:, :3], scene.texel_sites, scene.texel_height,
+                       scene.adjacency, scene.offsets, scene.faces, sh=coeffs)
+    rng = np.random.default_rng(1000 + args.seed)
+
+    def make(n):
+        reqs = []
+        for k in range(n):
+            w, h = res[int(rng.integers(0, 4))]
+            cam = workloads.sphere_cameras(rng, 1, w, h)[0]
+            reqs.append(RenderRequest(scene="s", camera=cam, rid=k))
+        return reqs
+
+    plain = PlainRenderer({"s": rs})
+    warm = make(8)
+    for r in warm:
+        plain.render(r)
+    torch.cuda.synchronize()
+    probe = make(16)
+    t0 = time.perf_counter()
+    for r in probe:
+        plain.render(r)
+        torch.cuda.synchronize()
+    mean_service = (time.perf_counter() - t0) / len(probe)
+    rate = args.serve_load / mean_service
+    trace = make(args.serve_requests)
+    arr = np.cumsum(rng.exponential(1.0 / rate, len(trace)))
+    for r, a in zip(trace, arr):
+        r.arrival = float(a)
+    rays = sum(r.width * r.height for r in trace)
+    log(f"serving: {len(trace)} requests, rate {rate:.1f}/s (plain mean service {mean_service * 1e3:.2f} ms)")
+
+    p_trace = copy.deepcopy(trace)
+    p_wall = plain.run(p_trace)
+    p_stats = latency_stats(p_trace)
+
+    srv = RenderServer("cuda", memory_budget=64 << 30, tile_capacity=4096, quantum=1024)
+    srv.add_scene("s", rs)
+    w_trace = make(4)
+    srv.run(w_trace)  # warm-up (graph capture)
+    s_trace = copy.deepcopy(trace)
+    srv.finished, srv.rejected = [], []
+    s_wall = srv.run(s_trace)
+    s_stats = latency_stats([r for r in s_trace if not r.rejected])
+    # identical images: the served result equals the plain path's
+    same = all(torch.equal(a.color, b.color) for a, b in zip(s_trace[:8], p_trace[:8]))
+    out = {
+        "requests": len(trace), "rate_hz": round(rate, 2), "load": args.serve_load,
+        "p50_ms": round(s_stats["p50_ms"], 3), "p99_ms": round(s_stats["p99_ms"], 3),
+        "mrays_per_s": round(rays / s_wall / 1e6, 2),
+        "plain_p50_ms": round(p_stats["p50_ms"], 3), "plain_p99_ms": round(p_stats["p99_ms"], 3),
+        "plain_mrays_per_s": round(rays / p_wall / 1e6, 2),
+        "rejected": len([r for r in s_trace if r.rejected]), "status": int(srv.status.item()),
+        "results_equal_plain": bool(same),
+        "workload": "1M-cell GMM scene resident, SH3 evaluated in the walk, forward requests, "
+                    "resolutions uniform over 256^2/512^2/1024^2/1920x1080, Poisson arrivals",
+    }
+    log(f"serving: p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms vs plain p50 {out['plain_p50_ms']} "
+        f"p99 {out['plain_p99_ms']} ms")
+    return out
 
 
 def cpu_baseline(sc, adj, off, faces, texel_rgb, cams_dev, fwd, gc, seconds):

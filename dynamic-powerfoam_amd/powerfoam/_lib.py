@@ -106,6 +106,16 @@ class PfCechGrid(ctypes.Structure):
     _fields_ = [("origin", c_float * 3), ("cell", c_float), ("dims", c_int32 * 3), ("reserved", c_int32)]
 
 
+class PfServeScene(ctypes.Structure):
+    _fields_ = [("scene", PfScene), ("sh", c_void_p), ("sh_degree", c_int32), ("reserved", c_int32)]
+
+
+class PfServeRequest(ctypes.Structure):
+    _fields_ = [("cam", PfCamera), ("scene", c_int32), ("mode", c_int32), ("tile_begin", c_int32),
+                ("tile_count", c_int32), ("step_tile0", c_int64), ("start_cell", c_void_p), ("color", c_void_p),
+                ("log_t", c_void_p), ("target", c_void_p), ("grads", PfGrads)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/pf_hip.h
 _SIGS = {
     "pf_device_count": (c_int32, [ctypes.POINTER(c_int32)]),
@@ -163,6 +173,7 @@ _SIGS = {
         [c_void_p, c_void_p, c_int64, ctypes.POINTER(PfCechGrid), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
          c_void_p],
     ),
+    "pf_serve_step": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_float, c_int32, c_void_p, c_void_p]),
     "pf_sh_forward": (
         c_int32,
         [c_void_p, c_int32, c_void_p, c_int64, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_int64, c_void_p],

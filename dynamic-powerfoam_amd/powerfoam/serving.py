@@ -1,0 +1,415 @@
+"""Serving: resident scenes, continuous batching over tiles, memory admission.
+
+``RenderServer`` keeps scenes resident in HBM and renders a stream of
+requests (one pinhole camera each, any resolution, forward or
+forward+backward against a target image).  Work advances in steps: every
+step is ONE fixed-shape launch of the serving kernel (``pf_serve_step``)
+over up to ``tile_capacity`` 8x8 tiles drawn from all active requests
+(round-robin, up to ``quantum`` tiles per request, leftovers to the
+oldest), so requests that arrive while others are on the GPU join at the
+next step and their tiles run in the same launch.  The request table lives
+in pinned host memory; each step copies it to the device and launches the
+kernel, and both are captured once in a HIP graph per table buffer (two
+buffers, so the host fills one while the GPU runs the other).
+
+Admission: a request reserves its output buffers (and gradient buffers for
+forward+backward) against ``memory_budget``; it waits while the budget is
+full and is rejected if it waited longer than ``max_wait`` seconds or can
+never fit.  ``PlainRenderer`` is the per-request drop-in path used as the
+latency baseline: SH colour pre-pass + walk (+ backward), one request at a
+time.
+"""
+import ctypes
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from .camera import cameras_to_device, pack_cameras
+from .raytrace import WalkScene, launch_walk_backward, launch_walk_forward, start_cells_device
+from .shading import sh_backward, sh_forward
+
+TILE = 8
+
+
+def tiles_of(w, h):
+    return ((w + TILE - 1) // TILE) * ((h + TILE - 1) // TILE)
+
+
+@dataclass
+class RenderRequest:
+    scene: str
+    camera: object  # TorchCamera
+    mode: str = "forward"  # or "forward_backward"
+    target: torch.Tensor = None  # (H, W, 3) for forward_backward
+    arrival: float = 0.0
+    rid: int = 0
+    # filled by the server
+    color: torch.Tensor = None
+    log_t: torch.Tensor = None
+    grads: dict = None
+    admitted: float = None
+    done: float = None
+    rejected: bool = False
+    _next_tile: int = 0
+    _bytes: int = 0
+    _cam_dev: torch.Tensor = None
+
+    @property
+    def width(self):
+        return self.camera.width
+
+    @property
+    def height(self):
+        return self.camera.height
+
+    @property
+    def ntiles(self):
+        return tiles_of(self.width, self.height)
+
+
+class ResidentScene:
+    """Device arrays of one scene (f32 attributes, f16 walk faces, K = 8)
+    plus optional degree-3 SH coefficients evaluated inside the walk."""
+
+    def __init__(self, points, radii, density, normals, texel_sites, texel_height, adjacency, adjacency_offsets,
+                 faces, sh=None, texel_rgb=None):
+        if sh is None and texel_rgb is None:
+            raise ValueError("a scene needs SH coefficients or texel_rgb")
+        K = texel_sites.shape[1]
+        if K != 8 or faces.dtype != torch.float16:
+            raise ValueError("served scenes use K = 8 texels and fp16 walk faces")
+        # with SH colour the kernel never reads texel_rgb; texel_sites (same
+        # shape) stands in so the scene struct holds a valid pointer
+        rgb = texel_rgb if texel_rgb is not None else texel_sites
+        self.walk = WalkScene(points, radii, density, normals, texel_sites, rgb, texel_height, adjacency,
+                              adjacency_offsets, faces)
+        self.points = points.detach().float().contiguous()
+        self.radii = radii.detach().float().contiguous()
+        self.sh = sh.detach().float().contiguous() if sh is not None else None
+        if self.sh is not None and self.sh.shape != (points.shape[0], 16, 3):
+            raise ValueError("SH coefficients must be (N, 16, 3) (degree 3)")
+        self.num_cells = points.shape[0]
+        self.K = K
+        s = _lib.PfServeScene()
+        s.scene = self.walk.c
+        s.sh = _lib.ptr(self.sh)
+        s.sh_degree = 3 if self.sh is not None else -1
+        self.c = s
+
+    def nbytes(self):
+        w = self.walk
+        tensors = [w.spheres, w.nsigmas, w.texel_sites, w.texel_height, w.adjacency, w.offsets, w.faces]
+        if self.sh is not None:
+            tensors.append(self.sh)
+        else:
+            tensors.append(w.texel_rgb)
+        return sum(t.numel() * t.element_size() for t in tensors)
+
+    def grad_bytes(self):
+        n, K = self.num_cells, self.K
+        return 4 * n * (4 + 4 + 3 * K + 3 * K + K)
+
+
+class RenderServer:
+    def __init__(self, device="cuda", memory_budget=64 << 30, tile_capacity=4096, quantum=1024, use_graph=True,
+                 max_wait=2.0, transmittance_threshold=1e-3, max_steps=1 << 16):
+        self.device = device
+        self.budget = int(memory_budget)
+        self.capacity = int(tile_capacity)
+        self.quantum = int(quantum)
+        self.use_graph = use_graph
+        self.max_wait = max_wait
+        self.thr = float(transmittance_threshold)
+        self.max_steps = int(max_steps)
+        self.scenes = {}
+        self.scene_index = {}
+        self.reserved = 0
+        self.queue = deque()
+        self.active = []
+        self.inflight = deque()  # (event, [requests completing in this step])
+        self.finished = []
+        self.rejected = []
+        self.nbuf = 2
+        self.max_reqs = 256
+        self._tables_dirty = True
+        self._graphs = None
+        self._step_count = 0
+        self.status = torch.zeros(1, dtype=torch.int32, device=device)
+        rq = ctypes.sizeof(_lib.PfServeRequest)
+        self._req_bytes = rq
+        self._host_tab = [torch.empty(self.max_reqs * rq + 16, dtype=torch.uint8, pin_memory=True)
+                          for _ in range(self.nbuf)]
+        self._dev_tab = [torch.empty(self.max_reqs * rq + 16, dtype=torch.uint8, device=device)
+                         for _ in range(self.nbuf)]
+        self._events = [None] * self.nbuf
+
+    # ------------------------------------------------------------ scenes --
+    def add_scene(self, name, scene: ResidentScene):
+        need = scene.nbytes()
+        if self.reserved + need > self.budget:
+            raise MemoryError(f"scene {name} ({need} B) does not fit in the memory budget")
+        self.reserved += need
+        self.scene_index[name] = len(self.scenes)
+        self.scenes[name] = scene
+        n = len(self.scenes)
+        arr = (_lib.PfServeScene * n)(*[s.c for s in self.scenes.values()])
+        raw = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8)
+        self._scene_tab = raw.to(self.device)
+        self._graphs = None  # the scene table pointer changed
+
+    # ---------------------------------------------------------- requests --
+    def _request_bytes(self, r):
+        b = r.width * r.height * 16
+        if r.mode == "forward_backward":
+            b += self.scenes[r.scene].grad_bytes() + r.width * r.height * 12
+        return b
+
+    def submit(self, r: RenderRequest):
+        r._bytes = self._request_bytes(r)
+        if r._bytes > self.budget - sum(s.nbytes() for s in self.scenes.values()):
+            r.rejected = True
+            self.rejected.append(r)
+            return False
+        self.queue.append(r)
+        return True
+
+    def _admit(self, now):
+        while self.queue and len(self.active) < self.max_reqs:
+            r = self.queue[0]
+            if self.reserved + r._bytes > self.budget:
+                if now - r.arrival > self.max_wait:
+                    self.queue.popleft()
+                    r.rejected = True
+                    self.rejected.append(r)
+                    continue
+                break
+            self.queue.popleft()
+            self.reserved += r._bytes
+            sc = self.scenes[r.scene]
+            dev = self.device
+            r.color = torch.empty(r.height, r.width, 3, dtype=torch.float32, device=dev)
+            r.log_t = torch.empty(r.height, r.width, dtype=torch.float32, device=dev)
+            if r.mode == "forward_backward":
+                n, K = sc.num_cells, sc.K
+                r.grads = dict(spheres=torch.zeros(n, 4, device=dev), nsigmas=torch.zeros(n, 4, device=dev),
+                               texel_sites=torch.zeros(n, K, 3, device=dev),
+                               texel_rgb=torch.zeros(n, K, 3, device=dev), texel_height=torch.zeros(n, K, device=dev))
+                r.target = r.target.to(dev, torch.float32).contiguous()
+            r._cam_dev = cameras_to_device([r.camera], None, dev)
+            start_cells_device(sc.points, sc.radii, r._cam_dev)
+            r.admitted = now
+            r._next_tile = 0
+            self.active.append(r)
+
+    def _schedule(self):
+        """Pick this step's tiles: up to quantum per active request in arrival
+        order, leftover capacity to the oldest requests."""
+        plan = []
+        cap = self.capacity
+        for r in self.active:
+            if cap <= 0:
+                break
+            take = min(r.ntiles - r._next_tile, self.quantum, cap)
+            if take > 0:
+                plan.append([r, r._next_tile, take])
+                cap -= take
+        for p in plan:
+            if cap <= 0:
+                break
+            r, b, take = p
+            more = min(r.ntiles - (b + take), cap)
+            if more > 0:
+                p[2] += more
+                cap -= more
+        return plan
+
+    def _write_table(self, buf, plan):
+        n = len(plan)
+        arr = (_lib.PfServeRequest * max(n, 1))()
+        t0 = 0
+        for k, (r, b, take) in enumerate(plan):
+            e = arr[k]
+            w = pack_cameras([r.camera])[0]
+            ctypes.memmove(ctypes.addressof(e.cam), w.ctypes.data, 48)
+            e.scene = self.scene_index[r.scene]
+            e.mode = 1 if r.mode == "forward_backward" else 0
+            e.tile_begin = b
+            e.tile_count = take
+            e.step_tile0 = t0
+            e.start_cell = r._cam_dev.data_ptr() + 36
+            e.color = r.color.data_ptr()
+            e.log_t = r.log_t.data_ptr()
+            if r.mode == "forward_backward":
+                e.target = r.target.data_ptr()
+                for key in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
+                    setattr(e.grads, key, r.grads[key].data_ptr())
+            t0 += take
+        host = self._host_tab[buf]
+        ctypes.memmove(host.data_ptr(), ctypes.addressof(arr), ctypes.sizeof(arr))
+        desc = np.array([n, t0], dtype=np.int64)
+        ctypes.memmove(host.data_ptr() + self.max_reqs * self._req_bytes, desc.ctypes.data, 16)
+        return t0
+
+    def _launch(self, buf):
+        lib = _lib.load()
+        host, dev = self._host_tab[buf], self._dev_tab[buf]
+        dev.copy_(host, non_blocking=True)
+        rc = lib.pf_serve_step(_lib.ptr(self._scene_tab), dev.data_ptr(),
+                               dev.data_ptr() + self.max_reqs * self._req_bytes, self.capacity, self.thr,
+                               self.max_steps, _lib.ptr(self.status), _lib.stream_ptr())
+        _lib.check(rc, "pf_serve_step")
+
+    def _build_graphs(self):
+        self._graphs = []
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            for buf in range(self.nbuf):  # warm up outside capture with an empty table
+                desc = np.zeros(2, np.int64)
+                ctypes.memmove(self._host_tab[buf].data_ptr() + self.max_reqs * self._req_bytes, desc.ctypes.data,
+                               16)
+                self._launch(buf)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        for buf in range(self.nbuf):
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._launch(buf)
+            self._graphs.append(g)
+        torch.cuda.synchronize()
+
+    def step(self, now=None):
+        """Run one step if there is work; returns the number of tiles."""
+        now = time.perf_counter() if now is None else now
+        self._admit(now)
+        if not self.active:
+            return 0
+        plan = self._schedule()
+        buf = self._step_count % self.nbuf
+        if self._events[buf] is not None:
+            self._events[buf].synchronize()  # that buffer's previous step has read its table
+        ntiles = self._write_table(buf, plan)
+        if self.use_graph:
+            if self._graphs is None:
+                self._build_graphs()
+            self._graphs[buf].replay()
+        else:
+            self._launch(buf)
+        ev = torch.cuda.Event()
+        ev.record()
+        self._events[buf] = ev
+        completing = []
+        for r, b, take in plan:
+            r._next_tile = b + take
+            if r._next_tile >= r.ntiles:
+                completing.append(r)
+        self.active = [r for r in self.active if r._next_tile < r.ntiles]
+        self.inflight.append((ev, completing))
+        self._step_count += 1
+        return ntiles
+
+    def poll(self, block=False):
+        while self.inflight:
+            ev, reqs = self.inflight[0]
+            if not block and not ev.query():
+                return
+            ev.synchronize()
+            t = time.perf_counter()
+            self.inflight.popleft()
+            for r in reqs:
+                r.done = t
+                self.reserved -= r._bytes
+                self.finished.append(r)
+
+    def finish_gradients(self, r):
+        """dL/dSH of a finished forward+backward request (SH backward of its
+        texel-colour gradient)."""
+        sc = self.scenes[r.scene]
+        if sc.sh is None or r.grads is None:
+            return None
+        return sh_backward(sc.sh, 3, sc.walk.texel_sites, r._cam_dev, r.grads["texel_rgb"][None])
+
+    def run(self, requests, max_inflight=2):
+        """Serve a trace of requests (arrival times in seconds from now)."""
+        t0 = time.perf_counter()
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        for r in pending:
+            r.arrival += t0
+        while pending or self.queue or self.active or self.inflight:
+            now = time.perf_counter()
+            while pending and pending[0].arrival <= now:
+                self.submit(pending.popleft())
+            if (self.active or self.queue) and len(self.inflight) < max_inflight:
+                if self.step(now) == 0 and self.queue and not self.active:
+                    pass
+            else:
+                self.poll(block=bool(self.inflight) and not pending and not self.queue and not self.active)
+                if not self.inflight and not self.active and not self.queue and pending:
+                    time.sleep(max(0.0, min(pending[0].arrival - time.perf_counter(), 0.001)))
+            self.poll()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+
+class _SceneView:
+    def __init__(self, c, num_cells, K, keep):
+        self.c, self.num_cells, self.K, self._keep = c, num_cells, K, keep
+
+
+class PlainRenderer:
+    """Per-request drop-in path: SH pre-pass + walk forward (+ backward and
+    SH backward), one request at a time, as a user of RayTracer would."""
+
+    def __init__(self, scenes, transmittance_threshold=1e-3):
+        self.scenes = scenes
+        self.thr = transmittance_threshold
+
+    def render(self, r: RenderRequest):
+        sc = self.scenes[r.scene]
+        dev = sc.points.device
+        cam = cameras_to_device([r.camera], None, dev)
+        start_cells_device(sc.points, sc.radii, cam)
+        w = sc.walk
+        if sc.sh is not None:
+            rgb = sh_forward(sc.sh, 3, w.texel_sites, cam)
+            c = _lib.PfScene.from_buffer_copy(w.c)
+            c.texel_rgb = rgb.data_ptr()
+            scene = _SceneView(c, w.num_cells, w.K, rgb)
+        else:
+            scene = w
+        out = launch_walk_forward(scene, cam, r.width, r.height, self.thr, want_depth=False)
+        r.color, r.log_t = out["color"][0], out["log_t"][0]
+        if r.mode == "forward_backward":
+            tgt = r.target.to(dev, torch.float32)
+            gc = 2.0 * (out["color"] - tgt[None])
+            r.grads = launch_walk_backward(scene, cam, r.width, r.height, out["color"], out["log_t"], gc, None,
+                                           self.thr)
+            if sc.sh is not None:
+                sh_backward(sc.sh, 3, w.texel_sites, cam, r.grads["texel_rgb"][None])
+        return r
+
+    def run(self, requests):
+        """FIFO over a trace; each request starts when it has arrived and the
+        previous one is done."""
+        t0 = time.perf_counter()
+        for r in sorted(requests, key=lambda q: q.arrival):
+            r.arrival += t0
+            while time.perf_counter() < r.arrival:
+                time.sleep(max(0.0, min(r.arrival - time.perf_counter(), 0.001)))
+            r.admitted = time.perf_counter()
+            self.render(r)
+            torch.cuda.synchronize()
+            r.done = time.perf_counter()
+        return time.perf_counter() - t0
+
+
+def latency_stats(requests):
+    lat = np.array([r.done - r.arrival for r in requests if r.done is not None])
+    if lat.size == 0:
+        return dict(p50_ms=None, p99_ms=None, n=0)
+    return dict(p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
+                mean_ms=float(lat.mean() * 1e3), n=int(lat.size))

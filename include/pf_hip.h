@@ -308,6 +308,47 @@ int pf_cech_write(const float *centers, const float *radii, int64_t num_cells,
                   const int32_t *grid_items, const int64_t *row_offsets,
                   int32_t *adjacency, int32_t *adjacency_offsets, void *stream);
 
+/* ---- serving ------------------------------------------------------------ */
+/* No reference counterpart: the reference has no serving path (its closest
+ * relative is the viewer's one-request render loop, viewer.py:514-592).     */
+
+/* A scene resident on the device: f32 attributes, f16 walk faces, K = 8.
+ * sh: (N,16,3) degree-3 coefficients evaluated per texel site inside the
+ * walk, or NULL to use scene.texel_rgb.                                    */
+typedef struct pf_serve_scene {
+    pf_scene scene;
+    const float *sh;
+    int32_t sh_degree;
+    int32_t reserved;
+} pf_serve_scene;
+
+/* One request's share of a step: tiles [tile_begin, tile_begin+tile_count)
+ * of its image, placed at [step_tile0, step_tile0+tile_count) of the step's
+ * tile space (entries sorted by step_tile0).  mode 0: forward (color,
+ * log_t); mode 1: also the backward of sum |color - target|^2 into grads.  */
+typedef struct pf_serve_request {
+    pf_camera cam;
+    int32_t scene;
+    int32_t mode;
+    int32_t tile_begin;
+    int32_t tile_count;
+    int64_t step_tile0;
+    const int32_t *start_cell;  /* device int holding the start cell, or
+                                   NULL to use cam.start_cell            */
+    float *color;
+    float *log_t;
+    const float *target;
+    pf_grads grads;
+} pf_serve_request;
+
+/* One serving step.  scenes, requests and desc are device memory; desc =
+ * {num_requests, step_tiles}.  The launch covers tile_capacity tiles
+ * regardless of desc, so it can be captured in a graph once and replayed.  */
+int pf_serve_step(const pf_serve_scene *scenes,
+                  const pf_serve_request *requests, const int64_t *desc,
+                  int64_t tile_capacity, float transmittance_threshold,
+                  int32_t max_steps, int32_t *status, void *stream);
+
 /* ---- colour pre-pass ---------------------------------------------------- */
 
 /* Spherical-harmonics colour per texel site and camera (degree 0..3,

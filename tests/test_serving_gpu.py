@@ -1,0 +1,117 @@
+"""Serving: each request's result is the same alone and in a mixed batch,
+the same as the plain per-request path, and the same whether steps run as
+captured graph replays or eager launches; admission rejects what cannot fit."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+from powerfoam import raytrace as rt
+from powerfoam import workloads
+from powerfoam.serving import PlainRenderer, RenderRequest, RenderServer, ResidentScene
+
+DEV = "cuda"
+
+
+def t(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+@pytest.fixture(scope="module")
+def served(scene4k):
+    sc = scene4k
+    rng = np.random.default_rng(12)
+    sh = (0.1 * rng.standard_normal((len(sc["radii"]), 16, 3))).astype(np.float32)
+    pts, rad = t(sc["points"]), t(sc["radii"])
+    adj, off = t(sc["adjacency"]), t(sc["offsets"])
+    faces = rt.walk_faces(pts, rad, adj, off)
+    scene = ResidentScene(pts, rad, t(sc["density"]), t(sc["normals"]), t(sc["texel_sites"]), t(sc["texel_height"]),
+                          adj, off, faces, sh=t(sh))
+    return scene
+
+
+def _requests(seed, n, with_bwd=True):
+    rng = np.random.default_rng(seed)
+    res = [(64, 64), (96, 80), (128, 128), (40, 24)]
+    reqs = []
+    for k in range(n):
+        w, h = res[k % len(res)]
+        cam = workloads.sphere_cameras(rng, 1, w, h)[0]
+        mode = "forward_backward" if (with_bwd and k % 3 == 2) else "forward"
+        tgt = torch.from_numpy(rng.uniform(0, 1, (h, w, 3)).astype(np.float32)) if mode != "forward" else None
+        reqs.append(RenderRequest(scene="s", camera=cam, mode=mode, target=tgt, rid=k))
+    return reqs
+
+
+pytestmark = pytest.mark.gpu
+
+
+def _drain(server, reqs):
+    for r in reqs:
+        server.submit(r)
+    while server.queue or server.active:
+        server.step()
+    server.poll(block=True)
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_alone_equals_mixed(served, use_graph):
+    alone = _requests(3, 6)
+    for r in alone:
+        srv = RenderServer(DEV, tile_capacity=64, quantum=16, use_graph=use_graph)
+        srv.add_scene("s", served)
+        _drain(srv, [r])
+    mixed = _requests(3, 6)
+    srv = RenderServer(DEV, tile_capacity=64, quantum=16, use_graph=use_graph)
+    srv.add_scene("s", served)
+    _drain(srv, mixed)
+    assert int(srv.status.item()) == 0
+    for a, b in zip(alone, mixed):
+        assert b.done is not None and not b.rejected
+        assert torch.equal(a.color, b.color) and torch.equal(a.log_t, b.log_t)
+        if a.mode == "forward_backward":
+            for k in ("spheres", "nsigmas", "texel_rgb"):
+                x, y = a.grads[k].cpu().numpy(), b.grads[k].cpu().numpy()
+                assert np.linalg.norm(x - y) <= 1e-4 * max(np.linalg.norm(x), 1e-30)
+
+
+def test_served_equals_plain_path(served):
+    reqs = _requests(5, 4)
+    srv = RenderServer(DEV, tile_capacity=128, quantum=32)
+    srv.add_scene("s", served)
+    _drain(srv, reqs)
+    plain = PlainRenderer({"s": served})
+    for r in _requests(5, 4):
+        plain.render(r)
+        q = next(x for x in reqs if x.rid == r.rid)
+        assert torch.equal(q.color, r.color)
+        if r.mode == "forward_backward":
+            x, y = q.grads["spheres"].cpu().numpy(), r.grads["spheres"].cpu().numpy()
+            assert np.linalg.norm(x - y) <= 1e-4 * np.linalg.norm(y)
+
+
+def test_graph_replays_match_eager_repeatedly(served):
+    outs = []
+    for use_graph in (False, True, True):
+        reqs = _requests(8, 5, with_bwd=False)
+        srv = RenderServer(DEV, tile_capacity=32, quantum=8, use_graph=use_graph)
+        srv.add_scene("s", served)
+        _drain(srv, reqs)
+        outs.append([r.color.clone() for r in reqs])
+        torch.cuda.synchronize()
+    for a, b in zip(outs[0], outs[1]):
+        assert torch.equal(a, b)
+    for a, b in zip(outs[0], outs[2]):
+        assert torch.equal(a, b)
+
+
+def test_memory_admission_rejects_what_cannot_fit(served):
+    srv = RenderServer(DEV, memory_budget=served.nbytes() + 64 * 64 * 16 + 1024)
+    srv.add_scene("s", served)
+    small = _requests(9, 1, with_bwd=False)[0]
+    big = RenderRequest(scene="s", camera=workloads.front_camera(1024, 1024))
+    assert srv.submit(small)
+    assert not srv.submit(big) and big.rejected
+    _drain(srv, [])
+    assert small.done is not None
