@@ -23,6 +23,7 @@ sys.path.insert(0, os.path.join(ROOT, "dynamic-powerfoam_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+GRAD_KEYS = ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height")
 
 
 def log(*a):
@@ -97,8 +98,8 @@ def main():
         if record:
             ev[1].record()
         gc = 2.0 * (f["color"] - target)
-        for v in grads.values():
-            v.zero_()
+        for k in GRAD_KEYS:
+            grads[k].zero_()
         g_coeffs.zero_()
         if record:
             ev[2].record()
@@ -132,8 +133,8 @@ def main():
     shading.sh_forward(coeffs, 3, sites, cams_dev, out=texel_rgb)
     f = rt.launch_walk_forward(scene, cams_dev, W, H, want_depth=False, want_sequences=True, want_bytes=True)
     gc = 2.0 * (f["color"] - target)
-    for v in grads.values():
-        v.zero_()
+    for k in GRAD_KEYS:
+        grads[k].zero_()
     b = rt.launch_walk_backward(scene, cams_dev, W, H, f["color"], f["log_t"], gc, None, grads=grads,
                                 want_bytes=True)
     torch.cuda.synchronize()

@@ -211,18 +211,21 @@ class Rasterizer:
         g_log_t = None
         if grad_opacity_in is not None:
             g_log_t = _c(-grad_opacity_in.detach().float() * torch.exp(st["log_t"]))
+        # every tensor whose pointer goes to the kernel stays referenced here
+        # until the launch is enqueued (incoming gradients are often expanded
+        # views, so .contiguous() makes new tensors)
+        g_nd = g(grad_normal_distance_in, (H, W)) if grad_normal_distance_in is not None else None
+        g_n = g(grad_normal_in, (H, W, 3)) if grad_normal_in is not None else None
+        gq = g(grad_quantile_depths_in, tuple(st["dq"].shape)) if st["dq"] is not None else None
+        g_ct = g(grad_contrib_in, (N,)) if grad_contrib_in is not None else None
         gin = _lib.PfRasterGradIn()
         gin.color = _lib.ptr(g_color)
         gin.log_t = _lib.ptr(g_log_t)
-        gin.normal_distance = _lib.ptr(g(grad_normal_distance_in, (H, W)) if grad_normal_distance_in is not None
-                                       else None)
-        gin.normal = _lib.ptr(g(grad_normal_in, (H, W, 3)) if grad_normal_in is not None else None)
-        gq = None
-        if st["dq"] is not None:
-            gq = g(grad_quantile_depths_in, tuple(st["dq"].shape))
-            gin.quantile_depths = _lib.ptr(gq)
-        gin.contrib = _lib.ptr(g(grad_contrib_in, (N,)) if grad_contrib_in is not None else None)
-        keep = [g_color, g_log_t, gq]
+        gin.normal_distance = _lib.ptr(g_nd)
+        gin.normal = _lib.ptr(g_n)
+        gin.quantile_depths = _lib.ptr(gq)
+        gin.contrib = _lib.ptr(g_ct)
+        keep = [g_color, g_log_t, g_nd, g_n, gq, g_ct]
         grads = dict(spheres=torch.zeros(N, 4, device=dev), nsigmas=torch.zeros(N, 4, device=dev),
                      texel_sites=torch.zeros(N, K, 3, device=dev), texel_rgb=torch.zeros(N, K, 3, device=dev),
                      texel_height=torch.zeros(N, K, device=dev))
