@@ -45,8 +45,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays shard over GPUs with the "
-              f"scene replicated, see DESIGN.md; this build runs and reports a single MI355X)", flush=True)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: rays would shard over GPUs with "
+              f"the scene replicated in each GPU's HBM, and only the single-MI355X path is built and reported",
+              flush=True)
         sys.exit(2)
 
     import numpy as np

@@ -255,6 +255,32 @@ template <typename A> __device__ __forceinline__ SceneDev<A> scene_dev(const pf_
                        s.adjacency, s.adjacency_offsets, s.faces, (int)s.num_cells, s.num_texel_sites};
 }
 
+// Brent's cycle detection on the walk state.  The next cell is a function of
+// the current cell alone (the exit face of a fixed ray), and termination a
+// function of log T, so a walk that comes back to a (cell, log T) pair it
+// held before repeats that loop forever; the reference's loop
+// (raytrace.py:163) never ends on such a ray.  Three fp16 face planes that
+// no longer meet in a common line make such loops.  The walk stops on the
+// repeat and reports PF_STATUS_CYCLE; a walk whose state never repeats takes
+// exactly the steps it took before, and is found within twice the length of
+// its tail plus loop.
+struct CycleGuard {
+    int cell = kNoCell;
+    float log_t = 0.0f;
+    uint32_t power = 1, lam = 1;
+    __device__ __forceinline__ bool repeats(int prim, float lt) {
+        if (prim == cell && lt == log_t) return true;
+        if (lam == power) {
+            cell = prim;
+            log_t = lt;
+            power <<= 1;
+            lam = 0;
+        }
+        ++lam;
+        return false;
+    }
+};
+
 struct RayFwd {
     f3 rgb;
     float log_t, depth;
@@ -272,10 +298,12 @@ __device__ __forceinline__ RayFwd walk_ray_fwd(const SceneDev<A> &s, const RGB &
     RayFwd out{mk(0, 0, 0), 0.0f, 0.0f, 0, 0, kFnvOffset, 0};
     bool depth_found = false;
     int prim = start;
+    CycleGuard guard;
     while (true) {
         const float trans = pf_expf(out.log_t);
         if (trans < thr || prim == kNoCell) break;
         if ((unsigned)prim >= (unsigned)s.N || out.visits >= max_steps) { out.status |= PF_STATUS_STEP_CAP; break; }
+        if (guard.repeats(prim, out.log_t)) { out.status |= PF_STATUS_CYCLE; break; }
         if (seq && out.visits < seq_cap) seq[out.visits] = prim;
         ++out.visits;
         out.hash = (out.hash ^ (uint64_t)(uint32_t)prim) * kFnvPrime;
@@ -359,10 +387,12 @@ __device__ __forceinline__ int walk_ray_bwd(const SceneDev<A> &s, const RGB &col
     float log_t = 0.0f;
     int prim = start;
     int visits = 0;
+    CycleGuard guard;
     while (true) {
         const float trans = pf_expf(log_t);
         if (trans < thr || prim == kNoCell) break;
         if ((unsigned)prim >= (unsigned)s.N || visits >= max_steps) return PF_STATUS_STEP_CAP;
+        if (guard.repeats(prim, log_t)) return PF_STATUS_CYCLE;
         ++visits;
 
         const float4 sp = s.spheres[prim];

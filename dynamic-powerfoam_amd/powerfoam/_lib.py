@@ -16,7 +16,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpf_hip.so")
 
 PF_OK, PF_ERR_ARG, PF_ERR_DTYPE, PF_ERR_HIP, PF_ERR_OVERFLOW = 0, 1, 2, 3, 4
-PF_STATUS_STEP_CAP, PF_STATUS_LIST_CAP = 1, 2
+PF_STATUS_STEP_CAP, PF_STATUS_LIST_CAP, PF_STATUS_CYCLE = 1, 2, 4
 PF_F32, PF_F16 = 0, 1
 PF_BWD_EXACT_TEXEL = 1
 
@@ -181,6 +181,16 @@ _SIGS = {
     "pf_sh_backward": (
         c_int32,
         [c_void_p, c_int32, c_void_p, c_int64, c_int32, c_void_p, c_int32, c_void_p, c_int64, c_void_p, c_void_p],
+    ),
+    "pf_sv_forward": (
+        c_int32,
+        [c_void_p, c_int64, c_void_p, c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_void_p,
+         c_void_p, c_void_p],
+    ),
+    "pf_sv_backward": (
+        c_int32,
+        [c_void_p, c_int64, c_void_p, c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_void_p,
+         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     ),
 }
 

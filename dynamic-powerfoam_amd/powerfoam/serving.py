@@ -292,10 +292,10 @@ class RenderServer:
         buf = self._step_count % self.nbuf
         if self._events[buf] is not None:
             self._events[buf].synchronize()  # that buffer's previous step has read its table
+        if self.use_graph and self._graphs is None:
+            self._build_graphs()  # its warm-up writes empty tables, so it goes before this step's table
         ntiles = self._write_table(buf, plan)
         if self.use_graph:
-            if self._graphs is None:
-                self._build_graphs()
             self._graphs[buf].replay()
         else:
             self._launch(buf)

@@ -41,8 +41,8 @@ def sh_backward(coeffs, degree, texel_sites, cams_dev, grad_rgb, grad_coeffs=Non
 
 class SHColorFn(torch.autograd.Function):
     """texel_rgb = SH(coeffs; texel site directions), differentiable in the
-    coefficients (the sites are detached, as color_fn.py:333 passes
-    texel_sites.detach() to the spherical-Voronoi pre-pass)."""
+    coefficients (the sites are detached, as powerfoam/scene.py:333-335
+    passes texel_sites.detach() to the spherical-Voronoi pre-pass)."""
 
     @staticmethod
     def forward(ctx, coeffs, degree, texel_sites, cams_dev):

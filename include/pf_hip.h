@@ -32,6 +32,10 @@ extern "C" {
 #define PF_STATUS_STEP_CAP    1   /* a walk hit max_steps (reference: none,
                                      raytrace.py:163 loops forever)         */
 #define PF_STATUS_LIST_CAP    2   /* a tile list exceeded its buffer         */
+#define PF_STATUS_CYCLE       4   /* a walk returned to a (cell, log T) state
+                                     it held before, so it would repeat
+                                     forever (raytrace.py:163 never ends on
+                                     such a ray); the ray stops there      */
 
 /* ---- dtypes ------------------------------------------------------------ */
 #define PF_F32 0
@@ -372,6 +376,33 @@ int pf_sh_backward(const float *coeffs, int32_t degree, const float *texel_sites
                    const pf_camera *cams, int32_t num_cams,
                    const float *grad_texel_rgb, int64_t grad_cam_stride,
                    float *grad_coeffs, void *stream);
+
+/* ---- spherical-Voronoi colour pre-pass ----------------------------------
+ * Replaces SphericalVoronoi._forward / spherical_voronoi_fwd_kernel
+ * (powerfoam/color_fn.py:54-101, 188-246).  Per texel site s (num_sites =
+ * N*K rows of texel_sites, f32): dir = normalize(s - eye); dir . forward <
+ * fov_cos_cutoff gives value 0.5 and weight sum 0, otherwise value =
+ * max(sum w_i v_i / sum w_i + 0.5, 0) with w_i = exp(-temp_i |dir - axis_i|).
+ * axes/values are (sv_dof, num_sites, 3), temps (sv_dof, num_sites), all
+ * of `dtype` (PF_F32 / PF_F16), as are value_out (num_sites,3) and
+ * wsum_out (num_sites).  eye/forward are host float[3].                   */
+int pf_sv_forward(const float *texel_sites, int64_t num_sites,
+                  const float *eye, const float *forward, float fov_cos_cutoff,
+                  const void *axes, const void *values, const void *temps,
+                  int32_t sv_dof, int32_t dtype, void *value_out,
+                  void *wsum_out, void *stream);
+
+/* Replaces SphericalVoronoi._backward / spherical_voronoi_bwd_kernel
+ * (powerfoam/color_fn.py:105-184, 253-315).  Writes (does not accumulate)
+ * the gradients of lobes with w_i / sum >= 1e-3 at in-view sites; the
+ * caller zeroes grad_axes / grad_values / grad_temps first.  No gradient
+ * flows to the sites, as in the reference.                               */
+int pf_sv_backward(const float *texel_sites, int64_t num_sites,
+                   const float *eye, const float *forward, float fov_cos_cutoff,
+                   const void *axes, const void *values, const void *temps,
+                   int32_t sv_dof, int32_t dtype, const void *wsum,
+                   const void *value, const void *grad_value, void *grad_axes,
+                   void *grad_values, void *grad_temps, void *stream);
 
 #ifdef __cplusplus
 }

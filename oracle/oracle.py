@@ -351,3 +351,41 @@ def cech(centers, radii):
     adj = np.zeros(max(total, 1), np.int32)
     lib.orc_cech(_p(centers), _p(radii), n, _p(off), _p(adj), total)
     return adj[:total], off
+
+
+def _sv_args(sites, eye, fwd, axes, vals, temps):
+    sites = _c(sites, np.float32).reshape(-1, 3)
+    axes, vals, temps = _c(axes, np.float32), _c(vals, np.float32), _c(temps, np.float32)
+    dof, n = temps.shape
+    if axes.shape != (dof, n, 3) or vals.shape != (dof, n, 3) or sites.shape[0] != n:
+        raise ValueError("lobe arrays must be (dof, n, 3) / (dof, n) with n texel sites")
+    return sites, _c(eye, np.float32), _c(fwd, np.float32), axes, vals, temps, dof, n
+
+
+def sv_forward(sites, eye, fwd, cutoff, axes, vals, temps, f64=False):
+    """Spherical-Voronoi colour per texel site -> (value (n,3), wsum (n,))."""
+    lib = load()
+    sites, eye, fwd, axes, vals, temps, dof, n = _sv_args(sites, eye, fwd, axes, vals, temps)
+    dt = np.float64 if f64 else np.float32
+    fn = lib.orc_sv_forward_f64 if f64 else lib.orc_sv_forward
+    fn.restype = None
+    fn.argtypes = [c_void_p, c_int64, c_void_p, c_void_p, ctypes.c_float] + [c_void_p] * 3 + [ctypes.c_int32] + \
+        [c_void_p] * 2
+    value, wsum = np.zeros((n, 3), dt), np.zeros(n, dt)
+    fn(_p(sites), n, _p(eye), _p(fwd), cutoff, _p(axes), _p(vals), _p(temps), dof, _p(value), _p(wsum))
+    return value, wsum
+
+
+def sv_backward(sites, eye, fwd, cutoff, axes, vals, temps, wsum, value, grad, f64=False):
+    """Gradients (axes, vals, temps) of sum(grad * value); unset lobes stay 0."""
+    lib = load()
+    sites, eye, fwd, axes, vals, temps, dof, n = _sv_args(sites, eye, fwd, axes, vals, temps)
+    dt = np.float64 if f64 else np.float32
+    fn = lib.orc_sv_backward_f64 if f64 else lib.orc_sv_backward
+    fn.restype = None
+    fn.argtypes = [c_void_p, c_int64, c_void_p, c_void_p, ctypes.c_float] + [c_void_p] * 3 + [ctypes.c_int32] + \
+        [c_void_p] * 6
+    ga, gv, gt = np.zeros_like(axes, dt), np.zeros_like(vals, dt), np.zeros_like(temps, dt)
+    fn(_p(sites), n, _p(eye), _p(fwd), cutoff, _p(axes), _p(vals), _p(temps), dof, _p(_c(wsum, dt)),
+       _p(_c(value, dt)), _p(_c(grad, np.float32)), _p(ga), _p(gv), _p(gt))
+    return ga, gv, gt

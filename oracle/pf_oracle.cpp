@@ -395,6 +395,24 @@ template <typename R> struct RayOut {
     int status;
 };
 
+// Stop rule for walks that would never end.  The next cell is a function of
+// the cell alone and termination a function of log T, so a return to an
+// earlier (cell, log T) state repeats forever (raytrace.py:163 has no exit
+// for it).  The stop happens at the step Brent's method finds the repeat:
+// the state is saved at steps 0, 1, 3, 7, ... (after 2^k - 1 moves) and the
+// walk stops when the current state equals the saved one.  Status bit 4.
+template <typename R> struct Brent {
+    int cell = -1;
+    R log_t = 0;
+    uint32_t power = 1, lam = 1;
+    bool repeats(int prim, R lt) {
+        if (prim == cell && lt == log_t) return true;
+        if (lam == power) { cell = prim; log_t = lt; power <<= 1; lam = 0; }
+        ++lam;
+        return false;
+    }
+};
+
 // The walk: raytrace.py:115-241 (loop :163-236, face loop :183-200,
 // texel :213-226, compositing :230-236); depth as rasterize.py:871-877.
 template <typename R, typename FaceLoad>
@@ -404,10 +422,12 @@ RayOut<R> walk_ray(const SceneR<R> &s, FaceLoad face, V3<R> o, V3<R> d, int star
     R log_t = 0;
     bool found = false;
     int prim = start;
+    Brent<R> guard;
     while (true) {
         const R trans = xexp(log_t);
         if (trans < thr || prim == kNoCell) break;
         if (prim < 0 || prim >= s.N || out.visits >= max_steps) { out.status |= 1; break; }
+        if (guard.repeats(prim, log_t)) { out.status |= 4; break; }
         if (seq && out.visits < seq_cap) seq[out.visits] = prim;
         out.visits++;
         out.hash = (out.hash ^ (uint64_t)(uint32_t)prim) * kFnvPrime;
@@ -465,10 +485,12 @@ int walk_ray_bwd(const SceneR<R> &s, FaceLoad face, V3<R> o, V3<R> d, int start,
     R log_t = 0;
     V3<R> rgb = mk<R>(0, 0, 0);
     int prim = start, visits = 0;
+    Brent<R> guard;
     while (true) {
         const R trans = xexp(log_t);
         if (trans < thr || prim == kNoCell) break;
         if (prim < 0 || prim >= s.N || visits >= max_steps) return 1;
+        if (guard.repeats(prim, log_t)) return 4;
         visits++;
         const R *sp = s.spheres + 4 * (int64_t)prim;
         const V3<R> c = mk(sp[0], sp[1], sp[2]);
@@ -1492,4 +1514,111 @@ extern "C" int64_t orc_cech(const float *centers, const float *radii, int64_t N,
     if (adj && total <= cap)
         for (int64_t i = 0; i < N; ++i) std::copy(rows[i].begin(), rows[i].end(), adj + offsets[i]);
     return total;
+}
+
+/* ---- spherical-Voronoi colour pre-pass ----------------------------------
+ * Restates spherical_voronoi_fwd_kernel / _bwd_kernel
+ * (powerfoam/color_fn.py:54-101, 105-184): per texel site, lobe-major lobe
+ * arrays (dof, n, 3) / (dof, n).  R = float follows the device arithmetic
+ * (fma dot products, correctly rounded sqrt/div, the shared exp); R = double
+ * is the exact form the finite-difference check uses.  Attribute storage is
+ * f32 here; the f16 variant converts on the caller's side.                */
+namespace {
+template <typename R> V3<R> sv_dir(const float *s, const float *eye) {
+    V3<R> v = mk<R>((R)s[0] - (R)eye[0], (R)s[1] - (R)eye[1], (R)s[2] - (R)eye[2]);
+    R l = len(v);
+    return l > (R)0 ? dv(v, l) : mk<R>(0, 0, 0);
+}
+template <typename R> bool sv_in_view(V3<R> d, const float *fwd, float cut) {
+    return !(dot(d, mk<R>((R)fwd[0], (R)fwd[1], (R)fwd[2])) < (R)cut);
+}
+template <typename R> V3<R> ld3r(const float *p) { return mk<R>((R)p[0], (R)p[1], (R)p[2]); }
+
+template <typename R>
+void sv_fwd(const float *sites, int64_t n, const float *eye, const float *fwd, float cut, const float *axes,
+            const float *vals, const float *temps, int dof, R *value, R *wsum) {
+    for (int64_t t = 0; t < n; ++t) {
+        V3<R> d = sv_dir<R>(sites + 3 * t, eye);
+        if (!sv_in_view(d, fwd, cut)) {
+            value[3 * t] = value[3 * t + 1] = value[3 * t + 2] = (R)0.5;
+            wsum[t] = 0;
+            continue;
+        }
+        R ws = 0;
+        V3<R> vs = mk<R>(0, 0, 0);
+        for (int i = 0; i < dof; ++i) {
+            const int64_t q = (int64_t)i * n + t;
+            R w = xexp(-(R)temps[q] * len(sub(d, ld3r<R>(axes + 3 * q))));
+            ws = ws + w;
+            vs = macc(vs, w, ld3r<R>(vals + 3 * q));
+        }
+        wsum[t] = ws;
+        V3<R> v = add(dv(vs, ws), mk<R>(0.5, 0.5, 0.5));
+        value[3 * t] = mx(v.x, (R)0);
+        value[3 * t + 1] = mx(v.y, (R)0);
+        value[3 * t + 2] = mx(v.z, (R)0);
+    }
+}
+
+template <typename R>
+void sv_bwd(const float *sites, int64_t n, const float *eye, const float *fwd, float cut, const float *axes,
+            const float *vals, const float *temps, int dof, const R *wsum, const R *value, const float *grad,
+            R *g_axes, R *g_vals, R *g_temps) {
+    for (int64_t t = 0; t < n; ++t) {
+        V3<R> d = sv_dir<R>(sites + 3 * t, eye);
+        if (!sv_in_view(d, fwd, cut)) continue;
+        const R ws = wsum[t];
+        V3<R> val = mk<R>(value[3 * t], value[3 * t + 1], value[3 * t + 2]);
+        V3<R> g = ld3r<R>(grad + 3 * t);
+        if (val.x <= 0) g.x = 0;
+        if (val.y <= 0) g.y = 0;
+        if (val.z <= 0) g.z = 0;
+        val = sub(val, mk<R>(0.5, 0.5, 0.5));
+        const R dp_out = dot(g, val);
+        for (int i = 0; i < dof; ++i) {
+            const int64_t q = (int64_t)i * n + t;
+            const V3<R> ax = ld3r<R>(axes + 3 * q), v = ld3r<R>(vals + 3 * q);
+            const R temp = (R)temps[q];
+            const V3<R> dd = sub(d, ax);
+            const R dist = len(dd);
+            const R w = xexp(-temp * dist);
+            if (w / ws < (R)1e-3f) continue;
+            const V3<R> gv = dv(scl(w, g), ws);
+            g_vals[3 * q] = gv.x;
+            g_vals[3 * q + 1] = gv.y;
+            g_vals[3 * q + 2] = gv.z;
+            const V3<R> dwds = dv(scl(temp * w, dd), dist);
+            const R dwdt = -dist * w;
+            const R dp = dot(g, v);
+            const V3<R> ga = dv(sclr(dwds, dp - dp_out), ws);
+            g_axes[3 * q] = ga.x;
+            g_axes[3 * q + 1] = ga.y;
+            g_axes[3 * q + 2] = ga.z;
+            g_temps[q] = (dwdt * (dp - dp_out)) / ws;
+        }
+    }
+}
+}  // namespace
+
+extern "C" void orc_sv_forward(const float *sites, int64_t n, const float *eye, const float *fwd, float cut,
+                               const float *axes, const float *vals, const float *temps, int32_t dof, float *value,
+                               float *wsum) {
+    sv_fwd<float>(sites, n, eye, fwd, cut, axes, vals, temps, dof, value, wsum);
+}
+extern "C" void orc_sv_forward_f64(const float *sites, int64_t n, const float *eye, const float *fwd, float cut,
+                                   const float *axes, const float *vals, const float *temps, int32_t dof,
+                                   double *value, double *wsum) {
+    sv_fwd<double>(sites, n, eye, fwd, cut, axes, vals, temps, dof, value, wsum);
+}
+extern "C" void orc_sv_backward(const float *sites, int64_t n, const float *eye, const float *fwd, float cut,
+                                const float *axes, const float *vals, const float *temps, int32_t dof,
+                                const float *wsum, const float *value, const float *grad, float *g_axes,
+                                float *g_vals, float *g_temps) {
+    sv_bwd<float>(sites, n, eye, fwd, cut, axes, vals, temps, dof, wsum, value, grad, g_axes, g_vals, g_temps);
+}
+extern "C" void orc_sv_backward_f64(const float *sites, int64_t n, const float *eye, const float *fwd, float cut,
+                                    const float *axes, const float *vals, const float *temps, int32_t dof,
+                                    const double *wsum, const double *value, const float *grad, double *g_axes,
+                                    double *g_vals, double *g_temps) {
+    sv_bwd<double>(sites, n, eye, fwd, cut, axes, vals, temps, dof, wsum, value, grad, g_axes, g_vals, g_temps);
 }

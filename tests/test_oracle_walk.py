@@ -247,3 +247,56 @@ def test_walk_backward_matches_fp64_central_differences(scene_textured):
     a, fd = np.array(a), np.array(fd)
     rel = np.linalg.norm(a - fd) / np.linalg.norm(fd)
     assert rel < 1e-3, (rel, list(zip(picks, a, fd)))
+
+
+# ------------------------------------------------------ walks that loop --
+def _loop_scene(hit_b):
+    """Three cells whose exit faces send the rays of a narrow 3x3 camera
+    A -> B -> C -> A (exit t = 1, 2, 1.5 along +y from the eye (0,-3,0)).
+    With hit_b, cell B's sphere (centre t = 1.5, r = 0.3) is hit, so every
+    pass through the loop attenuates and the walk ends on transmittance."""
+    K = 8
+    pts = np.array([[5, 5, 5], [0, -1.5, 0] if hit_b else [5, 5, 6], [5, 6, 5]], np.float32)
+    rad = np.array([0.01, 0.3 if hit_b else 0.01, 0.01], np.float32)
+    normals = np.tile(np.array([0, 0, 1], np.float32), (3, 1))
+    sites = np.repeat(pts[:, None, :], K, 1)
+    adj = np.array([1, 2, 0], np.int32)
+    off = np.array([0, 1, 2, 3], np.int32)
+    faces = np.array([[0, 1, 0, 1 - 3], [0, 1, 0, 2 - 3], [0, 1, 0, 1.5 - 3]], np.float32)
+    S = orc.Scene(pts, rad, np.ones(3, np.float32), normals, sites, np.full((3, K, 3), 0.5, np.float32),
+                  np.repeat(2 * rad[:, None], K, 1), adj, off, faces)
+    w = pack_cameras([workloads.front_camera(3, 3, hfov_deg=1.0)], [0])  # all 9 rays within 0.02 of the axis
+    return S, w
+
+
+def test_non_attenuating_loop_stops_at_brent_step():
+    S, w = _loop_scene(hit_b=False)
+    out = orc.walk_forward(S, w, seq_cap=16)
+    seq = out["seq"][0, 1, 1]
+    assert out["status"] == 4
+    # states A B C A B C: Brent saves A (step 0), C (step 2) and finds C at step 5
+    assert out["visits"][0, 1, 1] == 5 and list(seq[:5]) == [0, 1, 2, 0, 1]
+    assert out["log_t"][0, 1, 1] == 0
+    g = orc.walk_backward(S, w, out["color"], np.ones_like(out["color"]))
+    assert g["status"] == 4
+
+
+def test_attenuating_loop_is_walked_to_the_threshold():
+    """The reference's loop ends on such a ray, so the walk must not stop it."""
+    S, w = _loop_scene(hit_b=True)
+    out = orc.walk_forward(S, w, seq_cap=256)
+    v = out["visits"][0, 1, 1]
+    assert out["status"] == 0 and np.all(np.exp(out["log_t"]) < 1e-3)
+    assert v > 9 and list(out["seq"][0, 1, 1][:9]) == [0, 1, 2] * 3
+
+
+def test_large_scene_loops_only_where_state_repeats(scene4k):
+    """On a real scene the guard changes nothing: every ray's walk equals the
+    FP64 owner sequence test's walk, so no state repeats and no bit 4."""
+    sc = scene4k
+    faces = orc.walk_faces(sc["points"], sc["radii"], sc["adjacency"], sc["offsets"], f16=True)
+    out = orc.walk_forward(_scene(sc, faces), _cam_words(sc, workloads.front_camera(64, 64)), seq_cap=512)
+    assert out["status"] == 0
+    for s, n in zip(out["seq"].reshape(-1, 512), out["visits"].reshape(-1)):
+        s = s[:min(n, 512)]
+        assert len(np.unique(s)) == len(s)

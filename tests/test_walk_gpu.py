@@ -251,3 +251,26 @@ def test_sh_forward_backward_match_fp64():
     mask = (0.5 + np.einsum("bnkl,nlc->bnkc", Y, coeffs.astype(np.float64))) > 0
     ref_g = np.einsum("bnkl,bnkc->nlc", Y, g * mask)
     assert _rel(gc.cpu().numpy(), ref_g) < 1e-5
+
+
+@pytest.mark.parametrize("hit_b", [False, True])
+def test_loop_guard_matches_cpu(hit_b):
+    """Hand-built three-cell loop: without attenuation both sides stop at the
+    same step with status 4; with attenuation both walk it to the threshold."""
+    from test_oracle_walk import _loop_scene
+
+    S, words = _loop_scene(hit_b)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+    pts, rad = t(S.spheres[:, :3]), t(S.spheres[:, 3])
+    gs = rt.WalkScene(pts, rad, t(S.nsigmas[:, 3]), t(S.nsigmas[:, :3]), t(S.texel_sites), t(S.texel_rgb),
+                      t(S.texel_height), t(S.adjacency), t(S.offsets), t(S.faces))
+    g = rt.launch_walk_forward(gs, t(words), 3, 3, want_sequences=True, seq_cap=64)
+    torch.cuda.synchronize()
+    c = orc.walk_forward(S, words, seq_cap=64)
+    assert int(g["status"].item()) == c["status"] == (0 if hit_b else 4)
+    assert np.array_equal(g["visits"].cpu().numpy(), c["visits"])
+    assert np.array_equal(g["seq"].cpu().numpy(), c["seq"])
+    assert np.array_equal(g["log_t"].cpu().numpy(), c["log_t"])
+    gg = rt.launch_walk_backward(gs, t(words), 3, 3, g["color"], g["log_t"], torch.ones_like(g["color"]), None)
+    torch.cuda.synchronize()
+    assert int(gg["status"].item()) == (0 if hit_b else 4)
