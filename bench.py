@@ -42,7 +42,12 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--serve-requests", type=int, default=160, help="requests in the serving trace (0 = skip)")
     ap.add_argument("--serve-load", type=float, default=0.6, help="offered load as a fraction of plain capacity")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy 5 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: rays would shard over GPUs with "
@@ -276,6 +281,20 @@ def serve_bench(scene, coeffs, pts, rad, sc, args):
     log(f"serving: p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms vs plain p50 {out['plain_p50_ms']} "
         f"p99 {out['plain_p99_ms']} ms")
     return out
+
+
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain copy of known size: under a byte counter (FETCH_SIZE /
+    WRITE_SIZE) each launch should read and write nbytes, which gives the
+    counter's scale on this device."""
+    import torch
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "launches": reps}), flush=True)
 
 
 def cpu_baseline(sc, adj, off, faces, texel_rgb, cams_dev, fwd, gc, seconds):

@@ -48,6 +48,14 @@ template <typename A> __device__ __forceinline__ void ld4(const A *p, f3 &n, flo
 
 __device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
 
+// LDS access through an explicitly LDS-qualified pointer, so that it
+// compiles to ds_* instructions rather than flat ones.
+typedef __attribute__((address_space(3))) float lds_float;
+__device__ __forceinline__ lds_float *as_lds(float *p) { return (lds_float *)p; }
+__device__ __forceinline__ void lds_add(float *p, float v) {
+    __hip_atomic_fetch_add(as_lds(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // ---------------------------------------------------------- colour sources --
 template <typename A> struct MemRGBCell {
     const A *p;
@@ -142,12 +150,11 @@ __device__ __forceinline__ TexFwd texel_fwd(f3 o, f3 d, float t_near, f3 c, f3 n
 // (texture.py:118-119, :158-159) and the radius gets no gradient through
 // the texel weights.  exact == true keeps every texel and adds that radius
 // term, d w_k / d r = 2 temp w_k |q - s_k|^2 / r^3, into dL_dr.
-template <typename A, int KT, typename RGBCell>
+template <typename A, int KT, typename RGBCell, typename Sink>
 __device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 n, float r, const A *sites,
                                           const RGBCell &rgbs, const A *heights, int K, const TexFwd &fw,
-                                          f3 dL_drgb, float dL_dtsurf_in, float *g_sites, float *g_rgb,
-                                          float *g_height, f3 &dL_dc, f3 &dL_dn, float &dL_dtnear, bool exact,
-                                          float &dL_dr) {
+                                          f3 dL_drgb, float dL_dtsurf_in, const Sink &sink, f3 &dL_dc, f3 &dL_dn,
+                                          float &dL_dtnear, bool exact, float &dL_dr) {
     if constexpr (KT > 0) K = KT;
     dL_dc = mk(0, 0, 0);
     dL_dn = mk(0, 0, 0);
@@ -169,14 +176,10 @@ __device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 
         if (!exact && w / fw.wsum < 1e-3f) continue;
         const f3 dw_ds = divs(((2.0f * kTemp) * w) * dv, r2);
         const f3 grgb = divs(w * dL_drgb, fw.wsum);
-        if (grgb.x != 0.0f) atomic_add(g_rgb + 3 * i + 0, grgb.x);
-        if (grgb.y != 0.0f) atomic_add(g_rgb + 3 * i + 1, grgb.y);
-        if (grgb.z != 0.0f) atomic_add(g_rgb + 3 * i + 2, grgb.z);
+        sink.rgb(i, grgb);
         const float gdp = dot(dL_drgb, rgbs(i, s));
         const f3 gs = divs(dw_ds * (gdp - gdp_in), fw.wsum);
-        if (gs.x != 0.0f) atomic_add(g_sites + 3 * i + 0, gs.x);
-        if (gs.y != 0.0f) atomic_add(g_sites + 3 * i + 1, gs.y);
-        if (gs.z != 0.0f) atomic_add(g_sites + 3 * i + 2, gs.z);
+        sink.site(i, gs);
         dL_dq = dL_dq + (-1.0f * gs);
         if (exact) dL_dr += ((gdp - gdp_in) / fw.wsum) * ((((2.0f * kTemp) * w) * dot(dv, dv)) / (r2 * r));
     }
@@ -208,12 +211,10 @@ __device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 
             if (!exact && w / fw.wsum0 < 1e-3f) continue;
             const f3 dw_ds = divs(((2.0f * kTemp) * w) * dv, r2);
             const float gh = (w * dL_dheight) / fw.wsum0;
-            if (gh != 0.0f) atomic_add(g_height + i, gh);
+            sink.height(i, gh);
             const float hdp = dL_dheight * (float)heights[i];
             const f3 gs = divs(dw_ds * (hdp - hdp_in), fw.wsum0);
-            if (gs.x != 0.0f) atomic_add(g_sites + 3 * i + 0, gs.x);
-            if (gs.y != 0.0f) atomic_add(g_sites + 3 * i + 1, gs.y);
-            if (gs.z != 0.0f) atomic_add(g_sites + 3 * i + 2, gs.z);
+            sink.site(i, gs);
             dL_dq0 = dL_dq0 + (-1.0f * gs);
             if (exact) dL_dr += ((hdp - hdp_in) / fw.wsum0) * ((((2.0f * kTemp) * w) * dot(dv, dv)) / (r2 * r));
         }
@@ -367,6 +368,157 @@ struct GradDev {
     float *spheres, *nsigmas, *sites, *rgb, *height;
 };
 
+// Destination of one shaded segment's gradients for its own cell: the
+// sphere [c, r], [n, sigma] and the K texels' sites, colours and heights.
+// lds == nullptr adds straight into the global arrays; otherwise into the
+// cell's row of a wave's LDS accumulator (CellCache below), in the layout
+// sphere 0-3, nsigma 4-7, sites 8.., colours 8+3K.., heights 8+6K...
+// Zero contributions are skipped, as the reference's skipped texels add
+// nothing.
+struct CellSink {
+    float *lds;
+    float *gs, *gn, *gsite, *grgb, *gh;
+    int K;
+    __device__ __forceinline__ void add(int off, float *g, float v) const {
+        if (v == 0.0f) return;
+        if (lds) lds_add(lds + off, v);
+        else atomic_add(g, v);
+    }
+    __device__ __forceinline__ void sphere(f3 dc, float dr) const {
+        add(0, gs + 0, dc.x);
+        add(1, gs + 1, dc.y);
+        add(2, gs + 2, dc.z);
+        add(3, gs + 3, dr);
+    }
+    __device__ __forceinline__ void nsig(f3 dn, float dsigma) const {
+        add(4, gn + 0, dn.x);
+        add(5, gn + 1, dn.y);
+        add(6, gn + 2, dn.z);
+        add(7, gn + 3, dsigma);
+    }
+    __device__ __forceinline__ void site(int i, f3 v) const {
+        add(8 + 3 * i, gsite + 3 * i, v.x);
+        add(9 + 3 * i, gsite + 3 * i + 1, v.y);
+        add(10 + 3 * i, gsite + 3 * i + 2, v.z);
+    }
+    __device__ __forceinline__ void rgb(int i, f3 v) const {
+        add(8 + 3 * K + 3 * i, grgb + 3 * i, v.x);
+        add(9 + 3 * K + 3 * i, grgb + 3 * i + 1, v.y);
+        add(10 + 3 * K + 3 * i, grgb + 3 * i + 2, v.z);
+    }
+    __device__ __forceinline__ void height(int i, float v) const { add(8 + 6 * K + i, gh + i, v); }
+};
+
+// Per-lane policy: every lane sends its gradients straight to global memory
+// with atomics, and the walk loop may diverge freely.
+struct LaneCache {
+    static constexpr bool kWave = false;
+    __device__ __forceinline__ int assign(int) { return -1; }
+    __device__ __forceinline__ float *row(int) const { return nullptr; }
+    __device__ __forceinline__ void flush_all() {}
+};
+
+// Wave policy: a software cache of per-cell gradient rows in LDS shared by
+// the 64 lanes (one 8x8 tile) of a wave.  Neighbouring rays cross the same
+// cells, so a tile's shaded segments fall on far fewer cells than segments;
+// the rows absorb them with LDS atomics and each cell reaches global memory
+// with one 64-lane atomic per residency instead of one per segment.  Slots
+// are fully associative: lane j holds the id of the cell in slot j; each
+// step assigns slots to the distinct cells the lanes shade (one ballot
+// round per distinct cell), evicting first-in-first-out among slots not
+// used this step.  Lanes whose cell finds no slot (more distinct cells in a
+// step than slots) use global atomics for that segment.  Every lane of the
+// wave must call assign() and flush_all() together (uniform control flow).
+template <int S, int KT> struct CellCache {
+    static constexpr bool kWave = true;
+    static constexpr int kRow = 8 + 7 * KT;    // floats per cell row
+    static constexpr int kPitch = kRow + 1;    // odd pitch: rows start in distinct banks
+    static_assert(S <= 64 && kRow <= 64, "one slot per lane and one row element per lane");
+    float *acc;   // S * kPitch floats of LDS, zeroed
+    int owner;    // lane j < S: cell in slot j, or kNoCell
+    uint32_t rr;  // next slot to evict
+    GradDev G;
+
+    __device__ __forceinline__ void init(float *lds, const GradDev &g) {
+        acc = lds;
+        G = g;
+        const int lane = __lane_id();
+        owner = kNoCell;
+        rr = 0;
+        for (int k = lane; k < S * kPitch; k += 64) as_lds(acc)[k] = 0.0f;
+        __builtin_amdgcn_wave_barrier();
+    }
+    __device__ __forceinline__ float *row(int slot) const { return slot >= 0 ? acc + slot * kPitch : nullptr; }
+
+    // Adds slot s's row to cell's global gradients and clears it: lane k
+    // moves element k.
+    __device__ __forceinline__ void flush(int s, int cell) {
+        const int k = __lane_id();
+        if (k < kRow) {
+            lds_float *p = as_lds(acc) + s * kPitch + k;
+            const float v = *p;
+            *p = 0.0f;
+            if (v != 0.0f) {
+                const int64_t c = cell;
+                float *dst;
+                if (k < 4) dst = G.spheres + 4 * c + k;
+                else if (k < 8) dst = G.nsigmas + 4 * c + (k - 4);
+                else if (k < 8 + 3 * KT) dst = G.sites + 3 * KT * c + (k - 8);
+                else if (k < 8 + 6 * KT) dst = G.rgb + 3 * KT * c + (k - 8 - 3 * KT);
+                else dst = G.height + KT * c + (k - 8 - 6 * KT);
+                atomic_add(dst, v);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+
+    // cell: this lane's shaded cell this step, or kNoCell.  Returns the slot
+    // holding the cell's row, or -1 (use global atomics).
+    __device__ __forceinline__ int assign(int cell) {
+        constexpr uint64_t kMask = S == 64 ? ~0ull : ((1ull << S) - 1);
+        const int lane = __lane_id();
+        uint64_t pending = __ballot(cell != kNoCell);
+        uint64_t used = 0;
+        int mine = -1;
+        while (pending) {
+            const int lead = __ffsll((unsigned long long)pending) - 1;
+            const int lc = __builtin_amdgcn_readlane(cell, lead);
+            const uint64_t members = __ballot(cell == lc);
+            pending &= ~members;
+            const uint64_t hit = __ballot(owner == lc) & kMask;
+            int slot = -1;
+            if (hit) {
+                slot = __ffsll((unsigned long long)hit) - 1;
+            } else {
+                const uint64_t free = ~used & kMask;
+                if (free) {
+                    const uint64_t after = free & (~0ull << rr);
+                    slot = __ffsll((unsigned long long)(after ? after : free)) - 1;
+                    rr = slot + 1 == S ? 0 : slot + 1;
+                    const int old = __builtin_amdgcn_readlane(owner, slot);
+                    if (old != kNoCell) flush(slot, old);
+                    if (lane == slot) owner = lc;
+                }
+            }
+            if (slot >= 0) {
+                used |= 1ull << slot;
+                if ((members >> lane) & 1) mine = slot;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        return mine;
+    }
+
+    __device__ __forceinline__ void flush_all() {
+        uint64_t m = __ballot(owner != kNoCell) & (S == 64 ? ~0ull : ((1ull << S) - 1));
+        while (m) {
+            const int slot = __ffsll((unsigned long long)m) - 1;
+            m &= m - 1;
+            flush(slot, __builtin_amdgcn_readlane(owner, slot));
+        }
+    }
+};
+
 // Backward by checkpoint and replay.  The forward's per-ray colour C and
 // log transmittance are the checkpoint.  The ray replays its walk in
 // forward order with the forward's arithmetic (so the cell sequence and
@@ -375,11 +527,13 @@ struct GradDev {
 // which equals the reverse-order accumulation of the rasterizer backward
 // (rasterize.py:1371-1426).  The per-segment routing of dL/dt to the
 // sphere, a power face, or the texel plane mirrors rasterize.py:1296-1516.
-// Returns the status bits; adds the ray's algorithmic bytes to bytes.
-template <typename A, typename F, int KT, typename RGB>
+// live == false: the lane has no ray (or a zero incoming gradient) but
+// still takes part in a wave policy's collective steps.  Returns the
+// status bits; adds the ray's algorithmic bytes to bytes.
+template <typename A, typename F, int KT, typename RGB, typename Cache = LaneCache>
 __device__ __forceinline__ int walk_ray_bwd(const SceneDev<A> &s, const RGB &colors, f3 o, f3 d, int start, float thr,
                                             int max_steps, f3 Cfin, f3 g, float gL, const GradDev &G, bool exact,
-                                            uint64_t &bytes) {
+                                            uint64_t &bytes, bool live = true, Cache cache = Cache{}) {
     constexpr uint64_t kVisitBytes = WalkBytes<A, F>::visit, kFaceBytes = WalkBytes<A, F>::face,
                        kTexelBytes = WalkBytes<A, F>::texel;
     const int K = KT > 0 ? KT : s.K;
@@ -387,70 +541,97 @@ __device__ __forceinline__ int walk_ray_bwd(const SceneDev<A> &s, const RGB &col
     float log_t = 0.0f;
     int prim = start;
     int visits = 0;
+    int status = 0;
     CycleGuard guard;
     while (true) {
-        const float trans = pf_expf(log_t);
-        if (trans < thr || prim == kNoCell) break;
-        if ((unsigned)prim >= (unsigned)s.N || visits >= max_steps) return PF_STATUS_STEP_CAP;
-        if (guard.repeats(prim, log_t)) return PF_STATUS_CYCLE;
-        ++visits;
-
-        const float4 sp = s.spheres[prim];
-        const f3 c = mk(sp.x, sp.y, sp.z);
-        const float r = sp.w;
-        SphereHit sh = ray_sphere(o, d, c, r);
-        if (length(c - o) < 4.0f * r) sh.hit = false;
-        float tn = sh.tn, tf = sh.tf;
-        int tn_id = tn < 1e-8f ? -2 : -1;
-        int tf_id = tf < 1e-8f ? -2 : -1;
-
-        const int e0 = s.off[prim];
-        const int e1 = s.off[prim + 1];
-        bytes += kVisitBytes + (uint64_t)(e1 - e0) * kFaceBytes;
-        const int n_adj = e1 - e0;
-        int next = kNoCell;
-        float pt_far = 1e10f;
-        for (int e = e0; e < e1; ++e) {
-            const int nb = s.adj[e];
-            f3 fn;
-            float fw;
-            FaceT<F>::load(s.faces, e, fn, fw);
-            float t, dp;
-            ray_pface_diff(o, d, fn, fw, t, dp);
-            if (dp >= 0.0f && t < pt_far) { next = nb; pt_far = t; }
-            if (dp >= 0.0f && t < tf) tf_id = e - e0;
-            tf = dp >= 0.0f ? fmin_(t, tf) : tf;
-            if (dp < 0.0f && t > tn) tn_id = e - e0;
-            tn = dp < 0.0f ? fmax_(t, tn) : tn;
+        float trans = 0.0f;
+        if (live) {
+            trans = pf_expf(log_t);
+            if (trans < thr || prim == kNoCell) {
+                live = false;
+            } else if ((unsigned)prim >= (unsigned)s.N || visits >= max_steps) {
+                status = PF_STATUS_STEP_CAP;
+                live = false;
+            } else if (guard.repeats(prim, log_t)) {
+                status = PF_STATUS_CYCLE;
+                live = false;
+            }
         }
-        f3 nrm;
-        float sigma;
-        ld4(s.nsig + 4 * (int64_t)prim, nrm, sigma);
-        if (!sh.hit || tn > tf || sigma < 1e-3f) { prim = next; continue; }
+        if constexpr (Cache::kWave) {
+            if (!__any(live)) break;
+        } else {
+            if (!live) break;
+        }
 
-        const int64_t tbase = (int64_t)prim * K;
-        bytes += (uint64_t)K * kTexelBytes;
-        const float tn0 = tn;
-        const int tn0_id = tn_id, tf0_id = tf_id;
-        const auto rgbs = colors.cell(prim, K);
-        const TexFwd tx = texel_fwd<A, KT>(o, d, tn, c, nrm, r, s.sites + 3 * tbase, rgbs, s.height + tbase, K);
-        if (tx.dp >= 0.0f && tx.t_surf < tf) tf_id = n_adj;
-        tf = tx.dp >= 0.0f ? fmin_(tx.t_surf, tf) : tf;
-        if (tx.dp < 0.0f && tx.t_surf > tn) tn_id = n_adj;
-        tn = tx.dp < 0.0f ? fmax_(tx.t_surf, tn) : tn;
-        const int cell = prim;
-        prim = next;
-        const float dt = tf - tn;
-        if (!(dt > 0.0f)) continue;
+        // ---- replay one step; shade = this segment contributes
+        int cell = kNoCell;
+        f3 c = mk(0, 0, 0), nrm = mk(0, 0, 0);
+        float r = 1.0f, sigma = 0.0f, tn = 0.0f, tf = 0.0f, tn0 = 0.0f, dt = 0.0f, e_dlt = 1.0f, alpha = 0.0f;
+        int tn_id = -1, tf_id = -1, tn0_id = -1, tf0_id = -1, e0 = 0, n_adj = 0;
+        int64_t tbase = 0;
+        TexFwd tx{};
+        if (live) {
+            ++visits;
+            const float4 sp = s.spheres[prim];
+            c = mk(sp.x, sp.y, sp.z);
+            r = sp.w;
+            SphereHit sh = ray_sphere(o, d, c, r);
+            if (length(c - o) < 4.0f * r) sh.hit = false;
+            tn = sh.tn;
+            tf = sh.tf;
+            tn_id = tn < 1e-8f ? -2 : -1;
+            tf_id = tf < 1e-8f ? -2 : -1;
 
-        const float dlt = (-sigma) * dt;
-        const float e_dlt = pf_expf(dlt);
-        const float alpha = 1.0f - e_dlt;
-        rgb.x = __builtin_fmaf(tx.rgb.x * alpha, trans, rgb.x);
-        rgb.y = __builtin_fmaf(tx.rgb.y * alpha, trans, rgb.y);
-        rgb.z = __builtin_fmaf(tx.rgb.z * alpha, trans, rgb.z);
-        log_t += dlt;
+            e0 = s.off[prim];
+            const int e1 = s.off[prim + 1];
+            bytes += kVisitBytes + (uint64_t)(e1 - e0) * kFaceBytes;
+            n_adj = e1 - e0;
+            int next = kNoCell;
+            float pt_far = 1e10f;
+            for (int e = e0; e < e1; ++e) {
+                const int nb = s.adj[e];
+                f3 fn;
+                float fw;
+                FaceT<F>::load(s.faces, e, fn, fw);
+                float t, dp;
+                ray_pface_diff(o, d, fn, fw, t, dp);
+                if (dp >= 0.0f && t < pt_far) { next = nb; pt_far = t; }
+                if (dp >= 0.0f && t < tf) tf_id = e - e0;
+                tf = dp >= 0.0f ? fmin_(t, tf) : tf;
+                if (dp < 0.0f && t > tn) tn_id = e - e0;
+                tn = dp < 0.0f ? fmax_(t, tn) : tn;
+            }
+            ld4(s.nsig + 4 * (int64_t)prim, nrm, sigma);
+            if (sh.hit && !(tn > tf) && !(sigma < 1e-3f)) {
+                tbase = (int64_t)prim * K;
+                bytes += (uint64_t)K * kTexelBytes;
+                tn0 = tn;
+                tn0_id = tn_id;
+                tf0_id = tf_id;
+                tx = texel_fwd<A, KT>(o, d, tn, c, nrm, r, s.sites + 3 * tbase, colors.cell(prim, K),
+                                      s.height + tbase, K);
+                if (tx.dp >= 0.0f && tx.t_surf < tf) tf_id = n_adj;
+                tf = tx.dp >= 0.0f ? fmin_(tx.t_surf, tf) : tf;
+                if (tx.dp < 0.0f && tx.t_surf > tn) tn_id = n_adj;
+                tn = tx.dp < 0.0f ? fmax_(tx.t_surf, tn) : tn;
+                dt = tf - tn;
+                if (dt > 0.0f) {
+                    cell = prim;
+                    const float dlt = (-sigma) * dt;
+                    e_dlt = pf_expf(dlt);
+                    alpha = 1.0f - e_dlt;
+                    rgb.x = __builtin_fmaf(tx.rgb.x * alpha, trans, rgb.x);
+                    rgb.y = __builtin_fmaf(tx.rgb.y * alpha, trans, rgb.y);
+                    rgb.z = __builtin_fmaf(tx.rgb.z * alpha, trans, rgb.z);
+                    log_t += dlt;
+                }
+            }
+            prim = next;
+        }
 
+        // ---- gradients of the shaded segment
+        const int slot = cache.assign(cell);
+        if (cell == kNoCell) continue;
         bytes += 2ull * (32ull + (uint64_t)K * 28ull);  // read-modify-write of the cell's gradients
         const float dL_dLnext = dot(g, Cfin - rgb) + gL;
         const float dL_dalpha = dot(g, tx.rgb) * trans;
@@ -464,11 +645,18 @@ __device__ __forceinline__ int walk_ray_bwd(const SceneDev<A> &s, const RGB &col
         if (tn_id == n_adj) dL_dts = dL_dtn;
         else if (tf_id == n_adj) dL_dts = dL_dtf;
 
+        const CellSink sink{cache.row(slot),
+                            G.spheres + 4 * (int64_t)cell,
+                            G.nsigmas + 4 * (int64_t)cell,
+                            G.sites + 3 * tbase,
+                            G.rgb + 3 * tbase,
+                            G.height + tbase,
+                            K};
+        const auto rgbs = colors.cell(cell, K);
         f3 dL_dc, dL_dn;
         float dL_dtn0, dL_dr = 0.0f;
         texel_bwd<A, KT>(o, d, tn0, c, nrm, r, s.sites + 3 * tbase, rgbs, s.height + tbase, K, tx, dL_dcolor,
-                         dL_dts, G.sites + 3 * tbase, G.rgb + 3 * tbase, G.height + tbase, dL_dc, dL_dn, dL_dtn0,
-                         exact, dL_dr);
+                         dL_dts, sink, dL_dc, dL_dn, dL_dtn0, exact, dL_dr);
         if (tn_id != n_adj) dL_dtn0 += dL_dtn;
         for (int side = 0; side < 2; ++side) {
             const int tid = side == 0 ? tn0_id : tf0_id;
@@ -496,18 +684,11 @@ __device__ __forceinline__ int walk_ray_bwd(const SceneDev<A> &s, const RGB &col
                 bytes += 32;  // neighbour sphere gradient read-modify-write
             }
         }
-        float *gs = G.spheres + 4 * (int64_t)cell;
-        atomic_add(gs + 0, dL_dc.x);
-        atomic_add(gs + 1, dL_dc.y);
-        atomic_add(gs + 2, dL_dc.z);
-        atomic_add(gs + 3, dL_dr);
-        float *gn = G.nsigmas + 4 * (int64_t)cell;
-        if (dL_dn.x != 0.0f) atomic_add(gn + 0, dL_dn.x);
-        if (dL_dn.y != 0.0f) atomic_add(gn + 1, dL_dn.y);
-        if (dL_dn.z != 0.0f) atomic_add(gn + 2, dL_dn.z);
-        atomic_add(gn + 3, dL_dsigma);
+        sink.sphere(dL_dc, dL_dr);
+        sink.nsig(dL_dn, dL_dsigma);
     }
-    return 0;
+    cache.flush_all();
+    return status;
 }
 
 }  // namespace pf

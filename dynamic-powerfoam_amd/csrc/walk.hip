@@ -75,27 +75,42 @@ __global__ __launch_bounds__(256) void walk_fwd_kernel(WalkParams p) {
     if (r.status && p.out.status) atomicOr(p.out.status, r.status);
 }
 
+// Slots of the per-wave gradient cache (K = 8: 64-float rows, 12.5 KB of
+// LDS per wave, 50 KB per block, three blocks per CU).
+constexpr int kCacheSlots = 48;
+
 template <typename A, typename F, int KT>
 __global__ __launch_bounds__(256) void walk_bwd_kernel(WalkParams p) {
     const TileLane tl = tile_lane(p, 4);
-    if (!tl.valid) return;
-    const f3 g = p.g_color ? mk(p.g_color[3 * tl.ray], p.g_color[3 * tl.ray + 1], p.g_color[3 * tl.ray + 2])
-                           : mk(0, 0, 0);
-    const float gL = p.g_log_t ? p.g_log_t[tl.ray] : 0.0f;
+    constexpr bool kWave = KT == 8;
+    if (!kWave && !tl.valid) return;
+    const int64_t ray = tl.valid ? tl.ray : 0;
+    const f3 g = p.g_color && tl.valid ? mk(p.g_color[3 * ray], p.g_color[3 * ray + 1], p.g_color[3 * ray + 2])
+                                       : mk(0, 0, 0);
+    const float gL = p.g_log_t && tl.valid ? p.g_log_t[ray] : 0.0f;
     uint64_t bytes = 12 + (p.g_log_t ? 4 : 0);  // incoming gradients
+    const bool live = tl.valid && (g.x != 0.0f || g.y != 0.0f || g.z != 0.0f || gL != 0.0f);
+    const Cam cam = load_cam(p.cams + 12 * tl.b);
+    const f3 d = ray_dir(cam, tl.pix_i, tl.pix_j);
+    const SceneDev<A> s = scene_dev<A>(p.s);
+    const MemRGB<A> colors{reinterpret_cast<const A *>(p.s.texel_rgb) + (int64_t)tl.b * p.s.texel_rgb_cam_stride};
+    const f3 Cfin = live ? mk(p.ck_color[3 * ray], p.ck_color[3 * ray + 1], p.ck_color[3 * ray + 2]) : mk(0, 0, 0);
+    if (live) bytes += 12;  // checkpoint colour
+    const GradDev G{p.grads.spheres, p.grads.nsigmas, p.grads.texel_sites,
+                    p.grads.texel_rgb + (int64_t)tl.b * p.grads.texel_rgb_cam_stride, p.grads.texel_height};
     int st = 0;
-    if (g.x != 0.0f || g.y != 0.0f || g.z != 0.0f || gL != 0.0f) {
-        const Cam cam = load_cam(p.cams + 12 * tl.b);
-        const f3 d = ray_dir(cam, tl.pix_i, tl.pix_j);
-        const SceneDev<A> s = scene_dev<A>(p.s);
-        const MemRGB<A> colors{reinterpret_cast<const A *>(p.s.texel_rgb) + (int64_t)tl.b * p.s.texel_rgb_cam_stride};
-        const f3 Cfin = mk(p.ck_color[3 * tl.ray], p.ck_color[3 * tl.ray + 1], p.ck_color[3 * tl.ray + 2]);
-        bytes += 12;  // checkpoint colour
-        const GradDev G{p.grads.spheres, p.grads.nsigmas, p.grads.texel_sites,
-                        p.grads.texel_rgb + (int64_t)tl.b * p.grads.texel_rgb_cam_stride, p.grads.texel_height};
+    if constexpr (kWave) {
+        using Cache = CellCache<kCacheSlots, 8>;
+        __shared__ float rows[4][kCacheSlots * Cache::kPitch];
+        Cache cache;
+        cache.init(rows[threadIdx.x >> 6], G);
+        st = walk_ray_bwd<A, F, KT, MemRGB<A>, Cache>(s, colors, cam.eye, d, cam.start, p.thr, p.max_steps, Cfin, g,
+                                                      gL, G, p.exact_texel, bytes, live, cache);
+    } else if (live) {
         st = walk_ray_bwd<A, F, KT>(s, colors, cam.eye, d, cam.start, p.thr, p.max_steps, Cfin, g, gL, G,
                                     p.exact_texel, bytes);
     }
+    if (!tl.valid) return;
     if (p.alg_bytes) p.alg_bytes[tl.ray] = bytes;
     if (st && p.status) atomicOr(p.status, st);
 }

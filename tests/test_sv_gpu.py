@@ -19,6 +19,7 @@ DEV = "cuda"
 def _case(seed, n=4096, dof=6):
     rng = np.random.default_rng(seed)
     sites = rng.uniform(-1.0, 1.0, (n, 3)).astype(np.float32)
+    sites[: n // 8, 1] -= 4.0  # behind the eye at y = -3: out of view
     ax = rng.standard_normal((dof, n, 3)).astype(np.float32)
     ax /= np.linalg.norm(ax, axis=-1, keepdims=True)
     vals = (0.4 * rng.standard_normal((dof, n, 3))).astype(np.float32)
