@@ -24,19 +24,57 @@ struct alignas(8) h4 { _Float16 x, y, z, w; };
 
 template <typename F> struct FaceT;
 template <> struct FaceT<_Float16> {
-    __device__ static __forceinline__ void load(const void *p, int64_t e, f3 &n, float &w) {
-        const h4 v = reinterpret_cast<const h4 *>(p)[e];
+    using Raw = h4;  // 8 bytes, kept packed until used
+    __device__ static __forceinline__ Raw raw(const void *p, int64_t e) { return reinterpret_cast<const h4 *>(p)[e]; }
+    __device__ static __forceinline__ void decode(const Raw &v, f3 &n, float &w) {
         n = mk((float)v.x, (float)v.y, (float)v.z);
         w = (float)v.w;
     }
+    __device__ static __forceinline__ void load(const void *p, int64_t e, f3 &n, float &w) { decode(raw(p, e), n, w); }
 };
 template <> struct FaceT<float> {
-    __device__ static __forceinline__ void load(const void *p, int64_t e, f3 &n, float &w) {
-        const float4 v = reinterpret_cast<const float4 *>(p)[e];
+    using Raw = float4;
+    __device__ static __forceinline__ Raw raw(const void *p, int64_t e) {
+        return reinterpret_cast<const float4 *>(p)[e];
+    }
+    __device__ static __forceinline__ void decode(const Raw &v, f3 &n, float &w) {
         n = mk(v.x, v.y, v.z);
         w = v.w;
     }
+    __device__ static __forceinline__ void load(const void *p, int64_t e, f3 &n, float &w) { decode(raw(p, e), n, w); }
 };
+
+// Visits the faces e0 <= e < e1 of a CSR row in order, calling
+// fn(e, neighbour, normal, weight).  Faces are fetched kFaceBatch at a time
+// before any is used, so a row costs about deg / kFaceBatch memory round
+// trips instead of deg; the tail of the last batch re-reads face e1 - 1 (in
+// bounds) and is not passed to fn.
+#ifndef PF_FACE_BATCH
+#define PF_FACE_BATCH 4
+#endif
+constexpr int kFaceBatch = PF_FACE_BATCH;
+template <typename F, typename Fn>
+__device__ __forceinline__ void for_faces(const void *faces, const int *adj, int e0, int e1, Fn &&fn) {
+    for (int e = e0; e < e1; e += kFaceBatch) {
+        typename FaceT<F>::Raw raw[kFaceBatch];
+        int nb[kFaceBatch];
+#pragma unroll
+        for (int k = 0; k < kFaceBatch; ++k) {
+            const int ek = e + k < e1 ? e + k : e1 - 1;
+            raw[k] = FaceT<F>::raw(faces, ek);
+            nb[k] = adj[ek];
+        }
+#pragma unroll
+        for (int k = 0; k < kFaceBatch; ++k) {
+            if (e + k < e1) {
+                f3 n;
+                float w;
+                FaceT<F>::decode(raw[k], n, w);
+                fn(e + k, nb[k], n, w);
+            }
+        }
+    }
+}
 
 template <typename A> __device__ __forceinline__ f3 ld3(const A *p) {
     return mk((float)p[0], (float)p[1], (float)p[2]);
@@ -97,6 +135,19 @@ __device__ __forceinline__ TexFwd texel_fwd(f3 o, f3 d, float t_near, f3 c, f3 n
                                             const RGBCell &rgbs, const A *heights, int K) {
     TexFwd out;
     if constexpr (KT > 0) K = KT;
+    // With K fixed, all sites and heights are fetched before the first use
+    // (one memory round trip instead of one per texel) and stay in
+    // registers for both passes.
+    constexpr int kHeld = KT > 0 ? KT : 1;
+    f3 held_site[kHeld];
+    float held_h[kHeld];
+    if constexpr (KT > 0) {
+#pragma unroll
+        for (int i = 0; i < KT; ++i) {
+            held_site[i] = ld3(sites + 3 * i);
+            held_h[i] = (float)heights[i];
+        }
+    }
     ray_plane(o, d, c, n, 0.0f, out.t0, out.dp0);
     const float tq0 = out.dp0 >= 0.0f ? t_near : fmax_(t_near, out.t0);
     const f3 q0 = ray_at(o, tq0, d);
@@ -105,11 +156,11 @@ __device__ __forceinline__ TexFwd texel_fwd(f3 o, f3 d, float t_near, f3 c, f3 n
 #pragma unroll
     for (int i = 0; i < (KT > 0 ? KT : 16); ++i) {
         if (KT == 0 && i >= K) break;
-        const f3 s = ld3(sites + 3 * i);
+        const f3 s = KT > 0 ? held_site[KT > 0 ? i : 0] : ld3(sites + 3 * i);
         const f3 dv = q0 - s;
         const float dsq = dot(dv, dv) * inv_r2;
         const float w = pf_expf(-kTemp * dsq);
-        hs = __builtin_fmaf(w, (float)heights[i], hs);
+        hs = __builtin_fmaf(w, KT > 0 ? held_h[KT > 0 ? i : 0] : (float)heights[i], hs);
         ws = ws + w;
     }
     ws = fmax_(ws, 1e-20f);
@@ -123,7 +174,7 @@ __device__ __forceinline__ TexFwd texel_fwd(f3 o, f3 d, float t_near, f3 c, f3 n
 #pragma unroll
     for (int i = 0; i < (KT > 0 ? KT : 16); ++i) {
         if (KT == 0 && i >= K) break;
-        const f3 s = ld3(sites + 3 * i);
+        const f3 s = KT > 0 ? held_site[KT > 0 ? i : 0] : ld3(sites + 3 * i);
         const f3 dv = q - s;
         const float dsq = dot(dv, dv) * inv_r2;
         const float w = pf_expf(-kTemp * dsq);
@@ -310,6 +361,9 @@ __device__ __forceinline__ RayFwd walk_ray_fwd(const SceneDev<A> &s, const RGB &
         out.hash = (out.hash ^ (uint64_t)(uint32_t)prim) * kFnvPrime;
 
         const float4 sp = s.spheres[prim];
+        f3 nrm;
+        float sigma;
+        ld4(s.nsig + 4 * (int64_t)prim, nrm, sigma);  // fetched with the sphere, used after the faces
         const f3 c = mk(sp.x, sp.y, sp.z);
         const float r = sp.w;
         SphereHit sh = ray_sphere(o, d, c, r);
@@ -321,20 +375,13 @@ __device__ __forceinline__ RayFwd walk_ray_fwd(const SceneDev<A> &s, const RGB &
         out.bytes += kVisitBytes + (uint64_t)(e1 - e0) * kFaceBytes;
         int next = kNoCell;
         float pt_far = 1e10f;
-        for (int e = e0; e < e1; ++e) {
-            const int nb = s.adj[e];
-            f3 fn;
-            float fw;
-            FaceT<F>::load(s.faces, e, fn, fw);
+        for_faces<F>(s.faces, s.adj, e0, e1, [&](int, int nb, f3 fn, float fw) {
             float t, dp;
             ray_pface_diff(o, d, fn, fw, t, dp);
             if (dp >= 0.0f && t < pt_far) { next = nb; pt_far = t; }
             tf = dp >= 0.0f ? fmin_(t, tf) : tf;
             tn = dp < 0.0f ? fmax_(t, tn) : tn;
-        }
-        f3 nrm;
-        float sigma;
-        ld4(s.nsig + 4 * (int64_t)prim, nrm, sigma);
+        });
         if (!sh.hit || tn > tf || sigma < 1e-3f) { prim = next; continue; }
 
         const int64_t tbase = (int64_t)prim * K;
@@ -573,6 +620,7 @@ __device__ __forceinline__ int walk_ray_bwd(const SceneDev<A> &s, const RGB &col
         if (live) {
             ++visits;
             const float4 sp = s.spheres[prim];
+            ld4(s.nsig + 4 * (int64_t)prim, nrm, sigma);  // fetched with the sphere, used after the faces
             c = mk(sp.x, sp.y, sp.z);
             r = sp.w;
             SphereHit sh = ray_sphere(o, d, c, r);
@@ -588,11 +636,7 @@ __device__ __forceinline__ int walk_ray_bwd(const SceneDev<A> &s, const RGB &col
             n_adj = e1 - e0;
             int next = kNoCell;
             float pt_far = 1e10f;
-            for (int e = e0; e < e1; ++e) {
-                const int nb = s.adj[e];
-                f3 fn;
-                float fw;
-                FaceT<F>::load(s.faces, e, fn, fw);
+            for_faces<F>(s.faces, s.adj, e0, e1, [&](int e, int nb, f3 fn, float fw) {
                 float t, dp;
                 ray_pface_diff(o, d, fn, fw, t, dp);
                 if (dp >= 0.0f && t < pt_far) { next = nb; pt_far = t; }
@@ -600,8 +644,7 @@ __device__ __forceinline__ int walk_ray_bwd(const SceneDev<A> &s, const RGB &col
                 tf = dp >= 0.0f ? fmin_(t, tf) : tf;
                 if (dp < 0.0f && t > tn) tn_id = e - e0;
                 tn = dp < 0.0f ? fmax_(t, tn) : tn;
-            }
-            ld4(s.nsig + 4 * (int64_t)prim, nrm, sigma);
+            });
             if (sh.hit && !(tn > tf) && !(sigma < 1e-3f)) {
                 tbase = (int64_t)prim * K;
                 bytes += (uint64_t)K * kTexelBytes;
