@@ -42,6 +42,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--serve-requests", type=int, default=160, help="requests in the serving trace (0 = skip)")
     ap.add_argument("--serve-load", type=float, default=0.6, help="offered load as a fraction of plain capacity")
+    ap.add_argument("--serve-capacity", type=int, default=16384, help="8x8 tiles per serving step")
+    ap.add_argument("--serve-quantum", type=int, default=1024, help="tiles per request per step before leftovers")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -257,7 +259,7 @@ def serve_bench(scene, coeffs, pts, rad, sc, args):
     p_wall = plain.run(p_trace)
     p_stats = latency_stats(p_trace)
 
-    srv = RenderServer("cuda", memory_budget=64 << 30, tile_capacity=4096, quantum=1024)
+    srv = RenderServer("cuda", memory_budget=64 << 30, tile_capacity=args.serve_capacity, quantum=args.serve_quantum)
     srv.add_scene("s", rs)
     w_trace = make(4)
     srv.run(w_trace)  # warm-up (graph capture)
@@ -274,8 +276,9 @@ def serve_bench(scene, coeffs, pts, rad, sc, args):
         "plain_p50_ms": round(p_stats["p50_ms"], 3), "plain_p99_ms": round(p_stats["p99_ms"], 3),
         "plain_mrays_per_s": round(rays / p_wall / 1e6, 2),
         "rejected": len([r for r in s_trace if r.rejected]), "status": int(srv.status.item()),
+        "tile_capacity": args.serve_capacity, "quantum": args.serve_quantum, "colour": srv.colour,
         "results_equal_plain": bool(same),
-        "workload": "1M-cell GMM scene resident, SH3 evaluated in the walk, forward requests, "
+        "workload": "1M-cell GMM scene resident, SH3 colour pre-pass per request, forward requests, "
                     "resolutions uniform over 256^2/512^2/1024^2/1920x1080, Poisson arrivals",
     }
     log(f"serving: p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms vs plain p50 {out['plain_p50_ms']} "

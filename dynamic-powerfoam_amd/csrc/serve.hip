@@ -9,7 +9,9 @@
 // found by binary search over the table's step_tile0 prefix.  Forward
 // requests write colour and log transmittance; forward+backward requests
 // also replay the walk for the gradient of sum_pixels |C - target|^2
-// (checkpoint: the colour just computed, kept in registers).
+// (checkpoint: the colour just computed, kept in registers).  The two
+// kinds run in two launches of one step, so forward tiles are not held to
+// the occupancy of the backward's register footprint.
 #include "pf_device.h"
 #include "pf_hip.h"
 #include "pf_launch.h"
@@ -27,13 +29,40 @@ __device__ __forceinline__ int find_request(const pf_serve_request *reqs, int n,
     return lo;
 }
 
-template <int DEG>
+// One ray of a request: forward, and with BWD the replayed backward of
+// sum |C - target|^2 (the checkpoint is the colour just computed).
+template <bool BWD, typename RGB>
+__device__ __forceinline__ int serve_ray(const SceneDev<float> &s, const RGB &colors, const pf_serve_request &rq,
+                                         const Cam &cam, f3 d, int64_t pix, float thr, int max_steps) {
+    const RayFwd r = walk_ray_fwd<float, _Float16, 8>(s, colors, cam.eye, d, cam.start, thr, 0.5f, max_steps,
+                                                       nullptr, 0);
+    rq.color[3 * pix] = r.rgb.x;
+    rq.color[3 * pix + 1] = r.rgb.y;
+    rq.color[3 * pix + 2] = r.rgb.z;
+    if (rq.log_t) rq.log_t[pix] = r.log_t;
+    int st = r.status;
+    if constexpr (BWD) {
+        const f3 tg = mk(rq.target[3 * pix], rq.target[3 * pix + 1], rq.target[3 * pix + 2]);
+        const f3 gc = 2.0f * (r.rgb - tg);
+        const GradDev G{rq.grads.spheres, rq.grads.nsigmas, rq.grads.texel_sites, rq.grads.texel_rgb,
+                        rq.grads.texel_height};
+        uint64_t bytes = 0;
+        st |= walk_ray_bwd<float, _Float16, 8>(s, colors, cam.eye, d, cam.start, thr, max_steps, r.rgb, gc, 0.0f, G,
+                                               false, bytes);
+    }
+    return st;
+}
+
+// The table lists forward entries first: tiles [0, desc[2]) are forward
+// only and [desc[2], desc[1]) forward+backward.  BWD selects the range, so
+// forward-only tiles run in a kernel without the backward's registers.
+template <int DEG, bool BWD, bool SH>
 __global__ __launch_bounds__(256) void serve_kernel(const pf_serve_scene *scenes, const pf_serve_request *reqs,
                                                     const int64_t *desc, float thr, int max_steps, int *status) {
     const int nreq = (int)desc[0];
-    const int64_t step_tiles = desc[1];
-    const int64_t g = (int64_t)xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
-    if (g >= step_tiles || nreq <= 0) return;
+    const int64_t begin = BWD ? desc[2] : 0, end = BWD ? desc[1] : desc[2];
+    const int64_t g = begin + (int64_t)xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    if (g >= end || nreq <= 0) return;
     const int lane = threadIdx.x & 63;
     const int ri = __builtin_amdgcn_readfirstlane(find_request(reqs, nreq, g));
     const pf_serve_request &rq = reqs[ri];
@@ -49,44 +78,10 @@ __global__ __launch_bounds__(256) void serve_kernel(const pf_serve_scene *scenes
     if (rq.start_cell) cam.start = __builtin_amdgcn_readfirstlane(*rq.start_cell);
     const f3 d = ray_dir(cam, pi, pj);
     const SceneDev<float> s = scene_dev<float>(sc.scene);
-    int st = 0;
-    if (sc.sh) {
-        const SHRGB<DEG> colors{sc.sh, cam.eye};
-        const RayFwd r = walk_ray_fwd<float, _Float16, 8>(s, colors, cam.eye, d, cam.start, thr, 0.5f, max_steps,
-                                                           nullptr, 0);
-        rq.color[3 * pix] = r.rgb.x;
-        rq.color[3 * pix + 1] = r.rgb.y;
-        rq.color[3 * pix + 2] = r.rgb.z;
-        if (rq.log_t) rq.log_t[pix] = r.log_t;
-        st |= r.status;
-        if (rq.mode == 1) {
-            const f3 tg = mk(rq.target[3 * pix], rq.target[3 * pix + 1], rq.target[3 * pix + 2]);
-            const f3 gc = 2.0f * (r.rgb - tg);
-            const GradDev G{rq.grads.spheres, rq.grads.nsigmas, rq.grads.texel_sites, rq.grads.texel_rgb,
-                            rq.grads.texel_height};
-            uint64_t bytes = 0;
-            st |= walk_ray_bwd<float, _Float16, 8>(s, colors, cam.eye, d, cam.start, thr, max_steps, r.rgb, gc, 0.0f,
-                                                   G, false, bytes);
-        }
-    } else {
-        const MemRGB<float> colors{reinterpret_cast<const float *>(sc.scene.texel_rgb)};
-        const RayFwd r = walk_ray_fwd<float, _Float16, 8>(s, colors, cam.eye, d, cam.start, thr, 0.5f, max_steps,
-                                                           nullptr, 0);
-        rq.color[3 * pix] = r.rgb.x;
-        rq.color[3 * pix + 1] = r.rgb.y;
-        rq.color[3 * pix + 2] = r.rgb.z;
-        if (rq.log_t) rq.log_t[pix] = r.log_t;
-        st |= r.status;
-        if (rq.mode == 1) {
-            const f3 tg = mk(rq.target[3 * pix], rq.target[3 * pix + 1], rq.target[3 * pix + 2]);
-            const f3 gc = 2.0f * (r.rgb - tg);
-            const GradDev G{rq.grads.spheres, rq.grads.nsigmas, rq.grads.texel_sites, rq.grads.texel_rgb,
-                            rq.grads.texel_height};
-            uint64_t bytes = 0;
-            st |= walk_ray_bwd<float, _Float16, 8>(s, colors, cam.eye, d, cam.start, thr, max_steps, r.rgb, gc, 0.0f,
-                                                   G, false, bytes);
-        }
-    }
+    const float *rgb = rq.texel_rgb ? rq.texel_rgb : reinterpret_cast<const float *>(sc.scene.texel_rgb);
+    int st;
+    if (SH && !rq.texel_rgb && sc.sh) st = serve_ray<BWD>(s, SHRGB<DEG>{sc.sh, cam.eye}, rq, cam, d, pix, thr, max_steps);
+    else st = serve_ray<BWD>(s, MemRGB<float>{rgb}, rq, cam, d, pix, thr, max_steps);
     if (st && status) atomicOr(status, st);
 }
 
@@ -94,12 +89,26 @@ __global__ __launch_bounds__(256) void serve_kernel(const pf_serve_scene *scenes
 
 using namespace pf;
 
+template <bool SH>
+static void launch_step(const pf_serve_scene *scenes, const pf_serve_request *requests, const int64_t *desc,
+                        unsigned blocks, float thr, int32_t max_steps, int32_t *status, hipStream_t st) {
+    hipLaunchKernelGGL((serve_kernel<3, false, SH>), dim3(blocks), dim3(256), 0, st, scenes, requests, desc, thr,
+                       max_steps, status);
+    hipLaunchKernelGGL((serve_kernel<3, true, SH>), dim3(blocks), dim3(256), 0, st, scenes, requests, desc, thr,
+                       max_steps, status);
+}
+
 extern "C" int pf_serve_step(const pf_serve_scene *scenes, const pf_serve_request *requests, const int64_t *desc,
-                             int64_t tile_capacity, float transmittance_threshold, int32_t max_steps,
+                             int64_t tile_capacity, float transmittance_threshold, int32_t max_steps, int32_t flags,
                              int32_t *status, void *stream) {
     if (!scenes || !requests || !desc || tile_capacity <= 0 || max_steps <= 0) return PF_ERR_ARG;
+    if (flags & ~PF_SERVE_SH_IN_WALK) return PF_ERR_ARG;
     const unsigned blocks = (unsigned)((tile_capacity + 3) / 4);
-    hipLaunchKernelGGL(serve_kernel<3>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, scenes, requests, desc,
-                       transmittance_threshold, max_steps, status);
+    if (flags & PF_SERVE_SH_IN_WALK)
+        launch_step<true>(scenes, requests, desc, blocks, transmittance_threshold, max_steps, status,
+                          (hipStream_t)stream);
+    else
+        launch_step<false>(scenes, requests, desc, blocks, transmittance_threshold, max_steps, status,
+                           (hipStream_t)stream);
     return hipGetLastError() == hipSuccess ? PF_OK : PF_ERR_HIP;
 }

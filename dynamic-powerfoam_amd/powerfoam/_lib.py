@@ -19,6 +19,7 @@ PF_OK, PF_ERR_ARG, PF_ERR_DTYPE, PF_ERR_HIP, PF_ERR_OVERFLOW = 0, 1, 2, 3, 4
 PF_STATUS_STEP_CAP, PF_STATUS_LIST_CAP, PF_STATUS_CYCLE = 1, 2, 4
 PF_F32, PF_F16 = 0, 1
 PF_BWD_EXACT_TEXEL = 1
+PF_SERVE_SH_IN_WALK = 1
 
 c_void_p = ctypes.c_void_p
 c_int32 = ctypes.c_int32
@@ -113,7 +114,7 @@ class PfServeScene(ctypes.Structure):
 class PfServeRequest(ctypes.Structure):
     _fields_ = [("cam", PfCamera), ("scene", c_int32), ("mode", c_int32), ("tile_begin", c_int32),
                 ("tile_count", c_int32), ("step_tile0", c_int64), ("start_cell", c_void_p), ("color", c_void_p),
-                ("log_t", c_void_p), ("target", c_void_p), ("grads", PfGrads)]
+                ("log_t", c_void_p), ("target", c_void_p), ("grads", PfGrads), ("texel_rgb", c_void_p)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/pf_hip.h
@@ -173,7 +174,7 @@ _SIGS = {
         [c_void_p, c_void_p, c_int64, ctypes.POINTER(PfCechGrid), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
          c_void_p],
     ),
-    "pf_serve_step": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_float, c_int32, c_void_p, c_void_p]),
+    "pf_serve_step": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_float, c_int32, c_int32, c_void_p, c_void_p]),
     "pf_sh_forward": (
         c_int32,
         [c_void_p, c_int32, c_void_p, c_int64, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_int64, c_void_p],

@@ -9,13 +9,21 @@ over up to ``tile_capacity`` 8x8 tiles drawn from all active requests
 oldest), so requests that arrive while others are on the GPU join at the
 next step and their tiles run in the same launch.  The request table lives
 in pinned host memory; each step copies it to the device and launches the
-kernel, and both are captured once in a HIP graph per table buffer (two
-buffers, so the host fills one while the GPU runs the other).
+kernels, and both are captured once in a HIP graph per table buffer (two
+buffers, so the host fills one while the GPU runs the other; each buffer
+has its own stream, so one step's last tiles overlap the next step's
+first).
 
-Admission: a request reserves its output buffers (and gradient buffers for
-forward+backward) against ``memory_budget``; it waits while the budget is
-full and is rejected if it waited longer than ``max_wait`` seconds or can
-never fit.  ``PlainRenderer`` is the per-request drop-in path used as the
+Colour: with ``colour="prepass"`` (default) each admitted request of an
+SH-coloured scene gets its own texel colours from one SH pre-pass for its
+camera, which the step kernel then reads like the plain path does; with
+``colour="walk"`` the walk evaluates the SH at every shaded segment and no
+per-request colour buffer is needed (less memory, more arithmetic).
+
+Admission: a request reserves its output buffers (its colour buffer, and
+gradient buffers for forward+backward) against ``memory_budget``; it
+waits while the budget is full and is rejected if it waited longer than
+``max_wait`` seconds or can never fit.  ``PlainRenderer`` is the per-request drop-in path used as the
 latency baseline: SH colour pre-pass + walk (+ backward), one request at a
 time.
 """
@@ -57,6 +65,8 @@ class RenderRequest:
     _next_tile: int = 0
     _bytes: int = 0
     _cam_dev: torch.Tensor = None
+    _texel_rgb: torch.Tensor = None
+    _events: list = field(default_factory=list)
 
     @property
     def width(self):
@@ -73,7 +83,8 @@ class RenderRequest:
 
 class ResidentScene:
     """Device arrays of one scene (f32 attributes, f16 walk faces, K = 8)
-    plus optional degree-3 SH coefficients evaluated inside the walk."""
+    plus optional degree-3 SH coefficients (turned into texel colours by a
+    per-request pre-pass, or evaluated inside the walk)."""
 
     def __init__(self, points, radii, density, normals, texel_sites, texel_height, adjacency, adjacency_offsets,
                  faces, sh=None, texel_rgb=None):
@@ -115,8 +126,11 @@ class ResidentScene:
 
 
 class RenderServer:
-    def __init__(self, device="cuda", memory_budget=64 << 30, tile_capacity=4096, quantum=1024, use_graph=True,
-                 max_wait=2.0, transmittance_threshold=1e-3, max_steps=1 << 16):
+    def __init__(self, device="cuda", memory_budget=64 << 30, tile_capacity=16384, quantum=1024, use_graph=True,
+                 max_wait=2.0, transmittance_threshold=1e-3, max_steps=1 << 16, colour="prepass"):
+        if colour not in ("prepass", "walk"):
+            raise ValueError("colour must be 'prepass' or 'walk'")
+        self.colour = colour
         self.device = device
         self.budget = int(memory_budget)
         self.capacity = int(tile_capacity)
@@ -141,11 +155,15 @@ class RenderServer:
         self.status = torch.zeros(1, dtype=torch.int32, device=device)
         rq = ctypes.sizeof(_lib.PfServeRequest)
         self._req_bytes = rq
-        self._host_tab = [torch.empty(self.max_reqs * rq + 16, dtype=torch.uint8, pin_memory=True)
+        self._host_tab = [torch.empty(self.max_reqs * rq + 32, dtype=torch.uint8, pin_memory=True)
                           for _ in range(self.nbuf)]
-        self._dev_tab = [torch.empty(self.max_reqs * rq + 16, dtype=torch.uint8, device=device)
+        self._dev_tab = [torch.empty(self.max_reqs * rq + 32, dtype=torch.uint8, device=device)
                          for _ in range(self.nbuf)]
         self._events = [None] * self.nbuf
+        # one stream per table buffer (default priority): a step's tail overlaps
+        # the next step's start
+        self._streams = [torch.cuda.Stream(device=device) for _ in range(self.nbuf)]
+        self._flags = _lib.PF_SERVE_SH_IN_WALK if colour == "walk" else 0
 
     # ------------------------------------------------------------ scenes --
     def add_scene(self, name, scene: ResidentScene):
@@ -162,8 +180,14 @@ class RenderServer:
         self._graphs = None  # the scene table pointer changed
 
     # ---------------------------------------------------------- requests --
+    def _prepass(self, r):
+        return self.colour == "prepass" and self.scenes[r.scene].sh is not None
+
     def _request_bytes(self, r):
         b = r.width * r.height * 16
+        if self._prepass(r):
+            sc = self.scenes[r.scene]
+            b += sc.num_cells * sc.K * 12
         if r.mode == "forward_backward":
             b += self.scenes[r.scene].grad_bytes() + r.width * r.height * 12
         return b
@@ -201,6 +225,8 @@ class RenderServer:
                 r.target = r.target.to(dev, torch.float32).contiguous()
             r._cam_dev = cameras_to_device([r.camera], None, dev)
             start_cells_device(sc.points, sc.radii, r._cam_dev)
+            if self._prepass(r):  # this camera's texel colours, once, ahead of its tiles on the same stream
+                r._texel_rgb = sh_forward(sc.sh, 3, sc.walk.texel_sites, r._cam_dev)[0]
             r.admitted = now
             r._next_tile = 0
             self.active.append(r)
@@ -228,9 +254,13 @@ class RenderServer:
         return plan
 
     def _write_table(self, buf, plan):
+        """Forward entries first, then forward+backward ones (the step's
+        two launches split the tile space at the boundary)."""
+        plan = sorted(plan, key=lambda p: p[0].mode == "forward_backward")
         n = len(plan)
         arr = (_lib.PfServeRequest * max(n, 1))()
         t0 = 0
+        fwd_tiles = 0
         for k, (r, b, take) in enumerate(plan):
             e = arr[k]
             w = pack_cameras([r.camera])[0]
@@ -243,15 +273,18 @@ class RenderServer:
             e.start_cell = r._cam_dev.data_ptr() + 36
             e.color = r.color.data_ptr()
             e.log_t = r.log_t.data_ptr()
+            e.texel_rgb = r._texel_rgb.data_ptr() if r._texel_rgb is not None else 0
             if r.mode == "forward_backward":
                 e.target = r.target.data_ptr()
                 for key in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
                     setattr(e.grads, key, r.grads[key].data_ptr())
             t0 += take
+            if r.mode != "forward_backward":
+                fwd_tiles = t0
         host = self._host_tab[buf]
         ctypes.memmove(host.data_ptr(), ctypes.addressof(arr), ctypes.sizeof(arr))
-        desc = np.array([n, t0], dtype=np.int64)
-        ctypes.memmove(host.data_ptr() + self.max_reqs * self._req_bytes, desc.ctypes.data, 16)
+        desc = np.array([n, t0, fwd_tiles], dtype=np.int64)
+        ctypes.memmove(host.data_ptr() + self.max_reqs * self._req_bytes, desc.ctypes.data, 24)
         return t0
 
     def _launch(self, buf):
@@ -260,24 +293,23 @@ class RenderServer:
         dev.copy_(host, non_blocking=True)
         rc = lib.pf_serve_step(_lib.ptr(self._scene_tab), dev.data_ptr(),
                                dev.data_ptr() + self.max_reqs * self._req_bytes, self.capacity, self.thr,
-                               self.max_steps, _lib.ptr(self.status), _lib.stream_ptr())
+                               self.max_steps, self._flags, _lib.ptr(self.status), _lib.stream_ptr())
         _lib.check(rc, "pf_serve_step")
 
     def _build_graphs(self):
         self._graphs = []
-        s = torch.cuda.Stream()
-        s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            for buf in range(self.nbuf):  # warm up outside capture with an empty table
-                desc = np.zeros(2, np.int64)
+        for buf in range(self.nbuf):
+            s = self._streams[buf]
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):  # warm up outside capture with an empty table
+                desc = np.zeros(3, np.int64)
                 ctypes.memmove(self._host_tab[buf].data_ptr() + self.max_reqs * self._req_bytes, desc.ctypes.data,
-                               16)
+                               24)
                 self._launch(buf)
-        torch.cuda.current_stream().wait_stream(s)
         torch.cuda.synchronize()
         for buf in range(self.nbuf):
             g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
+            with torch.cuda.graph(g, stream=self._streams[buf]):
                 self._launch(buf)
             self._graphs.append(g)
         torch.cuda.synchronize()
@@ -295,16 +327,20 @@ class RenderServer:
         if self.use_graph and self._graphs is None:
             self._build_graphs()  # its warm-up writes empty tables, so it goes before this step's table
         ntiles = self._write_table(buf, plan)
-        if self.use_graph:
-            self._graphs[buf].replay()
-        else:
-            self._launch(buf)
-        ev = torch.cuda.Event()
-        ev.record()
+        s = self._streams[buf]
+        s.wait_stream(torch.cuda.current_stream())  # admission work (start cells, colour pre-pass)
+        with torch.cuda.stream(s):
+            if self.use_graph:
+                self._graphs[buf].replay()
+            else:
+                self._launch(buf)
+            ev = torch.cuda.Event()
+            ev.record(s)
         self._events[buf] = ev
         completing = []
         for r, b, take in plan:
             r._next_tile = b + take
+            r._events.append(ev)
             if r._next_tile >= r.ntiles:
                 completing.append(r)
         self.active = [r for r in self.active if r._next_tile < r.ntiles]
@@ -318,10 +354,15 @@ class RenderServer:
             if not block and not ev.query():
                 return
             ev.synchronize()
+            for r in reqs:  # its earlier steps may have run on the other stream
+                for e in r._events:
+                    e.synchronize()
             t = time.perf_counter()
             self.inflight.popleft()
             for r in reqs:
+                r._events = []
                 r.done = t
+                r._texel_rgb = None  # its last step has completed
                 self.reserved -= r._bytes
                 self.finished.append(r)
 

@@ -329,7 +329,11 @@ typedef struct pf_serve_scene {
 /* One request's share of a step: tiles [tile_begin, tile_begin+tile_count)
  * of its image, placed at [step_tile0, step_tile0+tile_count) of the step's
  * tile space (entries sorted by step_tile0).  mode 0: forward (color,
- * log_t); mode 1: also the backward of sum |color - target|^2 into grads.  */
+ * log_t); mode 1: also the backward of sum |color - target|^2 into grads.
+ * texel_rgb: this request's texel colours (N,K,3) f32 from a colour
+ * pre-pass for its camera (pf_sh_forward / pf_sv_forward), or NULL to use
+ * the scene's colour source (SH evaluated in the walk, or the scene's own
+ * texel_rgb).                                                             */
 typedef struct pf_serve_request {
     pf_camera cam;
     int32_t scene;
@@ -343,15 +347,25 @@ typedef struct pf_serve_request {
     float *log_t;
     const float *target;
     pf_grads grads;
+    const float *texel_rgb;
 } pf_serve_request;
 
 /* One serving step.  scenes, requests and desc are device memory; desc =
- * {num_requests, step_tiles}.  The launch covers tile_capacity tiles
- * regardless of desc, so it can be captured in a graph once and replayed.  */
+ * {num_requests, step_tiles, forward_tiles}: entries with mode 0 come first
+ * and cover step tiles [0, forward_tiles), mode-1 entries the rest.  Two
+ * launches (forward, forward+backward), each over tile_capacity tiles
+ * regardless of desc, so the step can be captured in a graph once and
+ * replayed.                                                               */
+#define PF_SERVE_SH_IN_WALK   1   /* flag of pf_serve_step: entries with no
+                                     texel_rgb of SH scenes evaluate the SH
+                                     in the walk (otherwise every entry reads
+                                     colours from memory: its own texel_rgb
+                                     or the scene's)                        */
 int pf_serve_step(const pf_serve_scene *scenes,
                   const pf_serve_request *requests, const int64_t *desc,
                   int64_t tile_capacity, float transmittance_threshold,
-                  int32_t max_steps, int32_t *status, void *stream);
+                  int32_t max_steps, int32_t flags, int32_t *status,
+                  void *stream);
 
 /* ---- colour pre-pass ---------------------------------------------------- */
 

@@ -107,7 +107,8 @@ def test_graph_replays_match_eager_repeatedly(served):
 
 
 def test_memory_admission_rejects_what_cannot_fit(served):
-    srv = RenderServer(DEV, memory_budget=served.nbytes() + 64 * 64 * 16 + 1024)
+    prepass = served.num_cells * served.K * 12  # the request's own texel colours
+    srv = RenderServer(DEV, memory_budget=served.nbytes() + 64 * 64 * 16 + prepass + 1024)
     srv.add_scene("s", served)
     small = _requests(9, 1, with_bwd=False)[0]
     big = RenderRequest(scene="s", camera=workloads.front_camera(1024, 1024))
@@ -115,3 +116,20 @@ def test_memory_admission_rejects_what_cannot_fit(served):
     assert not srv.submit(big) and big.rejected
     _drain(srv, [])
     assert small.done is not None
+
+
+def test_prepass_colour_equals_walk_colour(served):
+    """Texel colours from the per-request pre-pass and SH evaluated inside
+    the walk are the same numbers, so the images are identical."""
+    outs = []
+    for colour in ("prepass", "walk"):
+        reqs = _requests(21, 4)
+        srv = RenderServer(DEV, tile_capacity=64, quantum=16, colour=colour)
+        srv.add_scene("s", served)
+        _drain(srv, reqs)
+        outs.append(reqs)
+    for a, b in zip(*outs):
+        assert torch.equal(a.color, b.color) and torch.equal(a.log_t, b.log_t)
+        if a.mode == "forward_backward":
+            x, y = a.grads["texel_rgb"].cpu().numpy(), b.grads["texel_rgb"].cpu().numpy()
+            assert np.linalg.norm(x - y) <= 1e-4 * max(np.linalg.norm(x), 1e-30)
