@@ -178,7 +178,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (fp16 faces)",
+        "dtype": "f32",
         "data": "synthetic (seeded Gaussian-mixture scene, random target image)",
         "config": {
             "workload": f"walk fwd+bwd, {args.cells} cells GMM, {W}x{H}, SH3, L2 loss",
@@ -186,6 +186,8 @@ def main():
             "width": W,
             "height": H,
             "rays_per_step": rays,
+            "face_storage": "f16",
+            "attribute_storage": "f32",
             "mean_visits_per_ray": round(visits.mean().item(), 2),
             "walk_fwd_ms": round(fwd_ms, 4),
             "walk_bwd_ms": round(bwd_ms, 4),
@@ -200,7 +202,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic(dom) if (args.cells, W, H) == (1_000_000, 1920, 1080) else None,
         },
         "cpu_baseline": cpu,
         "serving": serving,
@@ -284,6 +286,23 @@ def serve_bench(scene, coeffs, pts, rad, sc, args):
     log(f"serving: p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms vs plain p50 {out['plain_p50_ms']} "
         f"p99 {out['plain_p99_ms']} ms")
     return out
+
+
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "walk_gmm1m_1080p_hbm_traffic.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of this workload, or None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        return None
+    return None
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):

@@ -161,7 +161,11 @@ __global__ __launch_bounds__(256) void bin_write_kernel(const float4 *spheres, i
 
 // Inclusive scan of n int32 counts into int64 offsets, one workgroup of
 // 1024 threads (tile counts: tens of thousands of entries).
-__global__ __launch_bounds__(1024) void scan_kernel(const int *in, int64_t n, int64_t *out) {
+// cap >= 0: offsets are clamped to cap (a caller-sized list buffer); a
+// total above cap sets PF_STATUS_LIST_CAP in *status, and every later
+// kernel reads only the clamped ranges.
+__global__ __launch_bounds__(1024) void scan_kernel(const int *in, int64_t n, int64_t *out, int64_t cap,
+                                                    int *status) {
     __shared__ int64_t part[1024];
     const int t = threadIdx.x;
     const int64_t per = (n + 1023) / 1024;
@@ -179,8 +183,9 @@ __global__ __launch_bounds__(1024) void scan_kernel(const int *in, int64_t n, in
     int64_t run = t > 0 ? part[t - 1] : 0;
     for (int64_t i = b; i < e; ++i) {
         run += in[i];
-        out[i] = run;
+        out[i] = cap >= 0 && run > cap ? cap : run;
     }
+    if (cap >= 0 && t == 1023 && part[1023] > cap && status) atomicOr(status, PF_STATUS_LIST_CAP);
 }
 
 // Per-tile sort of the 64-bit keys (power-distance bits, cell index):
@@ -249,9 +254,12 @@ __global__ __launch_bounds__(256) void tile_sort_kernel(const int64_t *offsets, 
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) prims[b + i] = (int)(uint32_t)(src[i] & 0xffffffffull);
 }
 
-__global__ void visible_mask_kernel(const int *prims, int64_t n, unsigned char *mask) {
+// Marks listed cells; reads the list length from the (possibly clamped)
+// offsets on the device, so n may be the buffer capacity.
+__global__ void visible_mask_kernel(const int *prims, int64_t n, const int64_t *offsets, int ntiles,
+                                    unsigned char *mask) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) mask[prims[i]] = 1;
+    if (i < n && i < offsets[ntiles]) mask[prims[i]] = 1;
 }
 
 // Raster faces: rasterize.py:580-606 — fp16 (p_j - p_i, r_j) per CSR edge.
@@ -922,7 +930,16 @@ extern "C" int pf_raster_bin_count(const float *spheres, int64_t num_cells, cons
 
 extern "C" int pf_raster_scan(const int32_t *counts, int64_t n, int64_t *offsets, void *stream) {
     if (!counts || !offsets || n <= 0) return PF_ERR_ARG;
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, counts, n, offsets);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, counts, n, offsets, (int64_t)-1,
+                       (int *)nullptr);
+    return hip_ok();
+}
+
+extern "C" int pf_raster_scan_bounded(const int32_t *counts, int64_t n, int64_t *offsets, int64_t capacity,
+                                      int32_t *status, void *stream) {
+    if (!counts || !offsets || n <= 0 || capacity < 0 || !status) return PF_ERR_ARG;
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, counts, n, offsets, capacity,
+                       status);
     return hip_ok();
 }
 
@@ -940,7 +957,7 @@ extern "C" int pf_raster_bin_write(const float *spheres, int64_t num_cells, cons
                        (unsigned long long *)keys, (unsigned long long *)scratch, tile_prims);
     if (visible_mask && num_isect > 0)
         hipLaunchKernelGGL(visible_mask_kernel, dim3((unsigned)((num_isect + 255) / 256)), dim3(256), 0, st,
-                           tile_prims, num_isect, visible_mask);
+                           tile_prims, num_isect, tile_offsets, num_tiles, visible_mask);
     return hip_ok();
 }
 

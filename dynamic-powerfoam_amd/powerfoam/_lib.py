@@ -139,6 +139,7 @@ _SIGS = {
     "pf_raster_faces": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
     "pf_raster_bin_count": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
     "pf_raster_scan": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p]),
+    "pf_raster_scan_bounded": (c_int32, [c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p]),
     "pf_raster_bin_write": (
         c_int32,
         [c_void_p, c_int64, c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,

@@ -64,9 +64,12 @@ def _scene_struct(spheres, nsigmas, sites, rgb, height, adjacency, offsets, face
     return s
 
 
-def bin_tiles(cam_dev, spheres, width, height, want_mask=True):
+def bin_tiles(cam_dev, spheres, width, height, want_mask=True, capacity=None, status=None):
     """Binning on the GPU: returns (offsets int64 (tiles+1), prims int32,
-    visible mask bool (N,)).  rasterize.py:1681-1772."""
+    visible mask bool (N,)).  rasterize.py:1681-1772.  With capacity (and
+    an int32 status tensor) the list buffer has that fixed size and nothing
+    is read back to the host; overflow sets PF_STATUS_LIST_CAP in status
+    and prims keeps the full capacity (valid up to offsets[-1])."""
     lib = _lib.load()
     dev = spheres.device
     tiles = ((height + 7) // 8) * ((width + 7) // 8)
@@ -76,8 +79,15 @@ def bin_tiles(cam_dev, spheres, width, height, want_mask=True):
     _lib.check(lib.pf_raster_bin_count(_lib.ptr(spheres), n, _lib.ptr(cam_dev), _lib.ptr(counts), st),
                "pf_raster_bin_count")
     offsets = torch.empty(tiles + 1, dtype=torch.int64, device=dev)
-    _lib.check(lib.pf_raster_scan(_lib.ptr(counts), tiles + 1, _lib.ptr(offsets), st), "pf_raster_scan")
-    n_isect = int(offsets[-1].item())
+    if capacity is not None:
+        if status is None:
+            raise ValueError("a bounded list needs a status tensor")
+        _lib.check(lib.pf_raster_scan_bounded(_lib.ptr(counts), tiles + 1, _lib.ptr(offsets), int(capacity),
+                                              _lib.ptr(status), st), "pf_raster_scan_bounded")
+        n_isect = int(capacity)
+    else:
+        _lib.check(lib.pf_raster_scan(_lib.ptr(counts), tiles + 1, _lib.ptr(offsets), st), "pf_raster_scan")
+        n_isect = int(offsets[-1].item())
     keys = torch.empty(max(n_isect, 1), dtype=torch.int64, device=dev)
     scratch = torch.empty(max(n_isect, 1), dtype=torch.int64, device=dev)
     prims = torch.empty(max(n_isect, 1), dtype=torch.int32, device=dev)
@@ -86,7 +96,7 @@ def bin_tiles(cam_dev, spheres, width, height, want_mask=True):
     _lib.check(lib.pf_raster_bin_write(_lib.ptr(spheres), n, _lib.ptr(cam_dev), tiles, _lib.ptr(offsets),
                                        _lib.ptr(wc), _lib.ptr(keys), _lib.ptr(scratch), _lib.ptr(prims),
                                        _lib.ptr(mask), n_isect, st), "pf_raster_bin_write")
-    return offsets, prims[:n_isect], (mask.bool() if want_mask else None)
+    return offsets, (prims if capacity is not None else prims[:n_isect]), (mask.bool() if want_mask else None)
 
 
 def raster_faces(spheres, adjacency, adjacency_offsets, dtype=torch.float16):
@@ -118,6 +128,13 @@ class Rasterizer:
         # and the gfx950 kernels always stage through LDS.
         self.disable_coop_prim_load = getattr(args, "disable_coop_prim_load", False)
         self.disable_coop_adj_load = getattr(args, "disable_coop_adj_load", False)
+        # Optional fixed size of the (tile, cell) list: binning then reads
+        # nothing back to the host; overflow shows in list_status
+        # (PF_STATUS_LIST_CAP) instead of resizing the buffer.
+        self.list_capacity = getattr(args, "raster_list_capacity", None)
+        self.list_status = None
+        if self.list_capacity is not None:
+            self.list_status = torch.zeros(1, dtype=torch.int32, device=device)
         _lib.load()
 
     # ---------------------------------------------------------- forward --
@@ -144,7 +161,8 @@ class Rasterizer:
         adj = _c(adjacency, torch.int32)
         off = _c(adjacency_offsets, torch.int32)
         cam_dev = cameras_to_device([camera], None, dev)
-        offsets, prims, mask = bin_tiles(cam_dev, spheres, W, H)
+        offsets, prims, mask = bin_tiles(cam_dev, spheres, W, H, capacity=self.list_capacity,
+                                         status=self.list_status)
         faces = raster_faces(spheres, adj, off)
         scene = _scene_struct(spheres, nsigmas, sites, rgb, height, adj, off, faces, _lib.PF_F32, _lib.PF_F16)
         color = torch.zeros(H, W, 3, dtype=torch.float32, device=dev)

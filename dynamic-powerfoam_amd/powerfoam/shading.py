@@ -33,8 +33,9 @@ def sh_backward(coeffs, degree, texel_sites, cams_dev, grad_rgb, grad_coeffs=Non
     B = cams_dev.shape[0]
     if grad_coeffs is None:
         grad_coeffs = torch.zeros_like(coeffs)
+    grad_rgb = grad_rgb.contiguous()  # held until the launch is enqueued
     rc = lib.pf_sh_backward(_lib.ptr(coeffs), degree, _lib.ptr(texel_sites), n, K, _lib.ptr(cams_dev), B,
-                            _lib.ptr(grad_rgb.contiguous()), n * K * 3, _lib.ptr(grad_coeffs), _lib.stream_ptr())
+                            _lib.ptr(grad_rgb), n * K * 3, _lib.ptr(grad_coeffs), _lib.stream_ptr())
     _lib.check(rc, "pf_sh_backward")
     return grad_coeffs
 

@@ -225,6 +225,15 @@ int pf_raster_bin_count(const float *spheres, int64_t num_cells,
 int pf_raster_scan(const int32_t *counts, int64_t n, int64_t *offsets,
                    void *stream);
 
+/* pf_raster_scan for a list buffer of fixed size, with no host read-back
+ * (the reference reads the total to the host, rasterize.py:1718): offsets
+ * are clamped to capacity, and a larger total sets PF_STATUS_LIST_CAP in
+ * *status (the tiles past the capacity then render incompletely).  Pass
+ * num_isect = capacity to pf_raster_bin_write; the whole raster pass is
+ * then free of host synchronisation and can be captured in a graph.       */
+int pf_raster_scan_bounded(const int32_t *counts, int64_t n, int64_t *offsets,
+                           int64_t capacity, int32_t *status, void *stream);
+
 /* Binning pass 2 (write_visible_pinhole_kernel, rasterize.py:293-363) and
  * the per-tile ordering (torch.argsort of the keys, :1771): each tile's
  * cells sorted by power distance from the eye (bit pattern of the positive

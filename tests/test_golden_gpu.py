@@ -83,10 +83,10 @@ def test_colour_prepass_matches_fixture(fx):
     from powerfoam import _lib
 
     sites = t(fx["texel_sites"].reshape(-1, 3))
+    ax, vals, temps = t(fx["sv_axes"]), t(fx["sv_values"]), t(fx["sv_temps"])  # held across the launch
     rc = _lib.load().pf_sv_forward(_lib.ptr(sites), sites.shape[0], eye.data_ptr(), fwd.data_ptr(),
-                                   float(fx["sv_cutoff"]), _lib.ptr(t(fx["sv_axes"])), _lib.ptr(t(fx["sv_values"])),
-                                   _lib.ptr(t(fx["sv_temps"])), sv.sv_dof, _lib.PF_F32, _lib.ptr(v), _lib.ptr(w),
-                                   _lib.stream_ptr())
+                                   float(fx["sv_cutoff"]), _lib.ptr(ax), _lib.ptr(vals), _lib.ptr(temps), sv.sv_dof,
+                                   _lib.PF_F32, _lib.ptr(v), _lib.ptr(w), _lib.stream_ptr())
     _lib.check(rc, "pf_sv_forward")
     torch.cuda.synchronize()
     assert np.array_equal(v.cpu().numpy(), fx["sv_value"]) and np.array_equal(w.cpu().numpy(), fx["sv_wsum"])
