@@ -2,6 +2,15 @@
 #include <hip/hip_runtime.h>
 #include "pf_hip.h"
 
+// The ABI's struct sizes, which bindings mirror field for field.
+static_assert(sizeof(pf_camera) == 48, "pf_camera layout");
+static_assert(sizeof(pf_scene) == 104, "pf_scene layout");
+static_assert(sizeof(pf_walk_out) == 72, "pf_walk_out layout");
+static_assert(sizeof(pf_grads) == 48, "pf_grads layout");
+static_assert(sizeof(pf_serve_scene) == 120, "pf_serve_scene layout");
+static_assert(sizeof(pf_serve_request) == 160, "pf_serve_request layout");
+static_assert(sizeof(pf_cech_grid) == 32, "pf_cech_grid layout");
+
 extern "C" int pf_device_count(int32_t *n) {
     if (!n) return PF_ERR_ARG;
     int c = 0;

@@ -39,9 +39,16 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     from powerfoam import _lib
 
-    assert ctypes.sizeof(_lib.PfCamera) == 48
-    assert ctypes.sizeof(_lib.PfScene) == 8 * 8 + 2 * 8 + 4 * 4 + 8
-    assert ctypes.sizeof(_lib.PfGrads) == 5 * 8 + 8
+    # the same sizes are static_asserts in csrc/capi.hip, so the library
+    # and this binding agree or the library does not build
+    sizes = dict(PfCamera=48, PfScene=104, PfWalkOut=72, PfGrads=48, PfServeScene=120, PfServeRequest=160,
+                 PfCechGrid=32)
+    for name, size in sizes.items():
+        assert ctypes.sizeof(getattr(_lib, name)) == size, name
+    src = open(os.path.join(ROOT, "dynamic-powerfoam_amd", "csrc", "capi.hip")).read()
+    for cname, size in (("pf_camera", 48), ("pf_scene", 104), ("pf_walk_out", 72), ("pf_grads", 48),
+                        ("pf_serve_scene", 120), ("pf_serve_request", 160), ("pf_cech_grid", 32)):
+        assert f"sizeof({cname}) == {size}" in src, cname
 
 
 def test_cpu_tensors_rejected_like_the_reference():
