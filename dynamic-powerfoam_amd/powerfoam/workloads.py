@@ -1,7 +1,7 @@
 """Synthetic scenes and cameras for the benchmark workloads.
 
 All randomness comes from ``numpy.random.default_rng(seed)`` drawn in a fixed
-order (sites, weights, density, colour, camera), so the CPU checker and the
+order (sites, weights, density, colour, camera), so the CPU oracle and the
 GPU see identical inputs.  A power weight w is read as w = r^2 (the
 reference's power is |x - p|^2 - r^2, powerfoam/rendering_math.py:79-84), so
 radii = sqrt(w).  Cells carry no dipole detail: normals (0, 0, 1), all K

@@ -134,8 +134,12 @@ int pf_walk_forward(const pf_scene *scene, const pf_camera *cams,
  * (raytrace.py:115-241) and no backward.  The
  * forward's per-ray color and log_t are the checkpoint; each ray's cell
  * sequence is replayed from its start cell and the gradients of
- * L(color, 1-exp(log_t)) are accumulated into grads with wave-aggregated
- * atomics.  grad_color (B,H,W,3), grad_log_t (B,H,W) (either may be NULL).
+ * L(color, 1-exp(log_t)) are accumulated into grads.  With K = 8 each
+ * 8x8 tile's wavefront keeps per-cell gradient rows in an LDS cache and
+ * adds a row to grads once per residency (one atomic per element); other
+ * K use per-segment atomics.  Summation order is not fixed, so gradients
+ * agree with a fixed-order sum to rounding, not bit for bit.
+ * grad_color (B,H,W,3), grad_log_t (B,H,W) (either may be NULL).
  * alg_bytes (B,H,W) or NULL: per-ray algorithmic bytes (the replay's reads
  * plus the gradient read-modify-writes).                                   */
 int pf_walk_backward(const pf_scene *scene, const pf_camera *cams,

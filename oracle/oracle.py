@@ -1,7 +1,7 @@
 """ctypes access to the CPU restatement ``oracle/build/libpf_oracle.so``.
 
 TEST INFRASTRUCTURE: only tests/, the smoke check and bench.py's CPU
-baseline use this module, as the checker.  The product path never imports
+baseline use this module, as the oracle.  The product path never imports
 it.  Arrays are numpy (host); f16 attribute arrays are widened to f32 by the
 caller (an exact conversion), f16 faces are passed as uint16 bits.
 """

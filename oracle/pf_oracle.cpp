@@ -1,6 +1,6 @@
 // pf_oracle.cpp — CPU restatement of the reference's power-diagram ray
 // tracing path.  TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check
-// and the bench's CPU baseline as the checker; nothing in the product path
+// and the bench's CPU baseline as the oracle; nothing in the product path
 // links or calls it.
 //
 // It follows the reference source as text (IsaacSuo/dynamic-powerfoam,
