@@ -44,36 +44,67 @@ template <> struct FaceT<float> {
     __device__ static __forceinline__ void load(const void *p, int64_t e, f3 &n, float &w) { decode(raw(p, e), n, w); }
 };
 
+// Packed walk faces (PF_F16_PACKED): the fp16 face and the neighbour index
+// of an edge in one 12-byte record.  A lane's face fetch is then one load
+// instead of two; the walk's lanes gather from different cells, so the
+// number of load instructions sets the L1 line traffic.
+struct PackedF16 {};
+struct PackedFace {
+    _Float16 x, y, z, w;
+    int32_t nb;
+};
+static_assert(sizeof(PackedFace) == 12, "packed face record");
+template <> struct FaceT<PackedF16> {
+    using Raw = PackedFace;
+    __device__ static __forceinline__ Raw raw(const void *p, int64_t e) {
+        return reinterpret_cast<const PackedFace *>(p)[e];
+    }
+    __device__ static __forceinline__ void decode(const Raw &v, f3 &n, float &w) {
+        n = mk((float)v.x, (float)v.y, (float)v.z);
+        w = (float)v.w;
+    }
+    __device__ static __forceinline__ void load(const void *p, int64_t e, f3 &n, float &w) { decode(raw(p, e), n, w); }
+};
 // Visits the faces e0 <= e < e1 of a CSR row in order, calling
-// fn(e, neighbour, normal, weight).  Faces are fetched kFaceBatch at a time
-// before any is used, so a row costs about deg / kFaceBatch memory round
-// trips instead of deg; the tail of the last batch re-reads face e1 - 1 (in
+// fn(e, normal, weight).  The neighbour across a face is not fetched here:
+// the walk needs it for the exit face only (face_neighbour after the scan),
+// and every load saved counts, since a wave's lanes gather from 64
+// different cells.  Faces are fetched kFaceBatch at a time before any is
+// used; the tail of the last batch re-reads the row's last face (in
 // bounds) and is not passed to fn.
 #ifndef PF_FACE_BATCH
 #define PF_FACE_BATCH 4
 #endif
 constexpr int kFaceBatch = PF_FACE_BATCH;
 template <typename F, typename Fn>
-__device__ __forceinline__ void for_faces(const void *faces, const int *adj, int e0, int e1, Fn &&fn) {
+__device__ __forceinline__ void for_faces(const void *faces, int64_t, int e0, int e1, Fn &&fn) {
     for (int e = e0; e < e1; e += kFaceBatch) {
         typename FaceT<F>::Raw raw[kFaceBatch];
-        int nb[kFaceBatch];
 #pragma unroll
-        for (int k = 0; k < kFaceBatch; ++k) {
-            const int ek = e + k < e1 ? e + k : e1 - 1;
-            raw[k] = FaceT<F>::raw(faces, ek);
-            nb[k] = adj[ek];
-        }
+        for (int k = 0; k < kFaceBatch; ++k) raw[k] = FaceT<F>::raw(faces, e + k < e1 ? e + k : e1 - 1);
 #pragma unroll
         for (int k = 0; k < kFaceBatch; ++k) {
             if (e + k < e1) {
                 f3 n;
                 float w;
                 FaceT<F>::decode(raw[k], n, w);
-                fn(e + k, nb[k], n, w);
+                fn(e + k, n, w);
             }
         }
     }
+}
+
+// The neighbour across face e (the adjacency entry, or the packed record's).
+template <typename F> __device__ __forceinline__ int face_neighbour(const void *, const int *adj, int64_t e) {
+    return adj[e];
+}
+template <> __device__ __forceinline__ int face_neighbour<PackedF16>(const void *faces, const int *, int64_t e) {
+    return reinterpret_cast<const PackedFace *>(faces)[e].nb;
+}
+
+template <typename F, typename Fn>
+__device__ __forceinline__ void scan_row(const void *faces, int64_t E, int e0, int e1, Fn &&fn) {
+    for_faces<F>(faces, E, e0, e1, fn);
 }
 
 template <typename A> __device__ __forceinline__ f3 ld3(const A *p) {
@@ -125,15 +156,19 @@ template <int DEG> struct SHRGB {
 // Two passes over the K texel sites: height at the base-plane query point,
 // then colour at the displaced plane's query point; weights use the walk's
 // r^-2 multiply (raytrace.py:61, :68).
-struct TexFwd {
+template <int KT> struct TexFwdT {
     float t0, dp0, height, wsum0, t_surf, dp, wsum;
     f3 rgb;
+    // K fixed: the texel weights of the height pass (w0) and the colour
+    // pass (w1), which the backward reuses instead of recomputing
+    float w0[KT > 0 ? KT : 1], w1[KT > 0 ? KT : 1];
 };
+using TexFwd = TexFwdT<0>;
 
 template <typename A, int KT, typename RGBCell>
-__device__ __forceinline__ TexFwd texel_fwd(f3 o, f3 d, float t_near, f3 c, f3 n, float r, const A *sites,
-                                            const RGBCell &rgbs, const A *heights, int K) {
-    TexFwd out;
+__device__ __forceinline__ TexFwdT<KT> texel_fwd(f3 o, f3 d, float t_near, f3 c, f3 n, float r, const A *sites,
+                                                 const RGBCell &rgbs, const A *heights, int K) {
+    TexFwdT<KT> out;
     if constexpr (KT > 0) K = KT;
     // With K fixed, all sites and heights are fetched before the first use
     // (one memory round trip instead of one per texel) and stay in
@@ -160,6 +195,7 @@ __device__ __forceinline__ TexFwd texel_fwd(f3 o, f3 d, float t_near, f3 c, f3 n
         const f3 dv = q0 - s;
         const float dsq = dot(dv, dv) * inv_r2;
         const float w = pf_expf(-kTemp * dsq);
+        if constexpr (KT > 0) out.w0[i] = w;
         hs = __builtin_fmaf(w, KT > 0 ? held_h[KT > 0 ? i : 0] : (float)heights[i], hs);
         ws = ws + w;
     }
@@ -178,6 +214,7 @@ __device__ __forceinline__ TexFwd texel_fwd(f3 o, f3 d, float t_near, f3 c, f3 n
         const f3 dv = q - s;
         const float dsq = dot(dv, dv) * inv_r2;
         const float w = pf_expf(-kTemp * dsq);
+        if constexpr (KT > 0) out.w1[i] = w;
         acc = macc(acc, w, rgbs(i, s));
         ws2 = ws2 + w;
     }
@@ -188,8 +225,8 @@ __device__ __forceinline__ TexFwd texel_fwd(f3 o, f3 d, float t_near, f3 c, f3 n
 }
 
 template <typename A, int KT>
-__device__ __forceinline__ TexFwd texel_fwd(f3 o, f3 d, float t_near, f3 c, f3 n, float r, const A *sites,
-                                            const A *rgbs, const A *heights, int K) {
+__device__ __forceinline__ TexFwdT<KT> texel_fwd(f3 o, f3 d, float t_near, f3 c, f3 n, float r, const A *sites,
+                                                 const A *rgbs, const A *heights, int K) {
     return texel_fwd<A, KT>(o, d, t_near, c, n, r, sites, MemRGBCell<A>{rgbs}, heights, K);
 }
 
@@ -203,7 +240,7 @@ __device__ __forceinline__ TexFwd texel_fwd(f3 o, f3 d, float t_near, f3 c, f3 n
 // term, d w_k / d r = 2 temp w_k |q - s_k|^2 / r^3, into dL_dr.
 template <typename A, int KT, typename RGBCell, typename Sink>
 __device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 n, float r, const A *sites,
-                                          const RGBCell &rgbs, const A *heights, int K, const TexFwd &fw,
+                                          const RGBCell &rgbs, const A *heights, int K, const TexFwdT<KT> &fw,
                                           f3 dL_drgb, float dL_dtsurf_in, const Sink &sink, f3 &dL_dc, f3 &dL_dn,
                                           float &dL_dtnear, bool exact, float &dL_dr) {
     if constexpr (KT > 0) K = KT;
@@ -211,28 +248,33 @@ __device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 
     dL_dn = mk(0, 0, 0);
     float dL_dheight = 0.0f;
     const float inv_r2 = 1.0f / (r * r);
-    const float r2 = r * r;
 
     const float tq = fw.dp >= 0.0f ? t_near_in : fmax_(t_near_in, fw.t_surf);
     const f3 q = ray_at(o, tq, d);
     f3 dL_dq = mk(0, 0, 0);
     const float gdp_in = dot(dL_drgb, fw.rgb);
+    // Gradient arithmetic multiplies by reciprocals (one division each per
+    // segment); the 1e-3 skip keeps its exact division, so the set of
+    // contributing texels is the forward's.
+    const float k2 = (2.0f * kTemp) * inv_r2;  // d w / d s = k2 w (q - s)
+    const float inv_ws = 1.0f / fw.wsum;
 #pragma unroll
     for (int i = 0; i < (KT > 0 ? KT : 16); ++i) {
         if (KT == 0 && i >= K) break;
         const f3 s = ld3(sites + 3 * i);
         const f3 dv = q - s;
-        const float dsq = dot(dv, dv) * inv_r2;
-        const float w = pf_expf(-kTemp * dsq);
+        float w;
+        if constexpr (KT > 0) w = fw.w1[i];
+        else w = pf_expf(-kTemp * (dot(dv, dv) * inv_r2));
         if (!exact && w / fw.wsum < 1e-3f) continue;
-        const f3 dw_ds = divs(((2.0f * kTemp) * w) * dv, r2);
-        const f3 grgb = divs(w * dL_drgb, fw.wsum);
-        sink.rgb(i, grgb);
+        const float kw = k2 * w;
+        sink.rgb(i, (w * inv_ws) * dL_drgb);
         const float gdp = dot(dL_drgb, rgbs(i, s));
-        const f3 gs = divs(dw_ds * (gdp - gdp_in), fw.wsum);
+        const float coef = (gdp - gdp_in) * inv_ws;
+        const f3 gs = (kw * coef) * dv;
         sink.site(i, gs);
         dL_dq = dL_dq + (-1.0f * gs);
-        if (exact) dL_dr += ((gdp - gdp_in) / fw.wsum) * ((((2.0f * kTemp) * w) * dot(dv, dv)) / (r2 * r));
+        if (exact) dL_dr += coef * (kw * dot(dv, dv) / r);
     }
     const float dL_dtq = dot(dL_dq, d);
     dL_dtnear = (fw.dp >= 0.0f || t_near_in >= fw.t_surf) ? dL_dtq : 0.0f;
@@ -252,22 +294,24 @@ __device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 
     f3 dL_dq0 = mk(0, 0, 0);
     const float hdp_in = dL_dheight * fw.height;
     if (dL_dheight != 0.0f) {
+        const float inv_ws0 = 1.0f / fw.wsum0;
 #pragma unroll
         for (int i = 0; i < (KT > 0 ? KT : 16); ++i) {
             if (KT == 0 && i >= K) break;
             const f3 s = ld3(sites + 3 * i);
             const f3 dv = q0 - s;
-            const float dsq = dot(dv, dv) * inv_r2;
-            const float w = pf_expf(-kTemp * dsq);
+            float w;
+            if constexpr (KT > 0) w = fw.w0[i];
+            else w = pf_expf(-kTemp * (dot(dv, dv) * inv_r2));
             if (!exact && w / fw.wsum0 < 1e-3f) continue;
-            const f3 dw_ds = divs(((2.0f * kTemp) * w) * dv, r2);
-            const float gh = (w * dL_dheight) / fw.wsum0;
-            sink.height(i, gh);
+            const float kw = k2 * w;
+            sink.height(i, (w * dL_dheight) * inv_ws0);
             const float hdp = dL_dheight * (float)heights[i];
-            const f3 gs = divs(dw_ds * (hdp - hdp_in), fw.wsum0);
+            const float coef = (hdp - hdp_in) * inv_ws0;
+            const f3 gs = (kw * coef) * dv;
             sink.site(i, gs);
             dL_dq0 = dL_dq0 + (-1.0f * gs);
-            if (exact) dL_dr += ((hdp - hdp_in) / fw.wsum0) * ((((2.0f * kTemp) * w) * dot(dv, dv)) / (r2 * r));
+            if (exact) dL_dr += coef * (kw * dot(dv, dv) / r);
         }
     }
     const float dL_dtq0 = dot(dL_dq0, d);
@@ -292,6 +336,7 @@ template <typename A, typename F> struct WalkBytes {
     static constexpr uint64_t face = 4 + 4 * sizeof(F);
     static constexpr uint64_t texel = 7 * sizeof(A);
 };
+template <typename A> struct WalkBytes<A, PackedF16> : WalkBytes<A, _Float16> {};
 
 template <typename A> struct SceneDev {
     const float4 *spheres;
@@ -299,12 +344,14 @@ template <typename A> struct SceneDev {
     const int *adj, *off;
     const void *faces;
     int N, K;
+    int64_t E;
 };
 
 template <typename A> __device__ __forceinline__ SceneDev<A> scene_dev(const pf_scene &s) {
     return SceneDev<A>{reinterpret_cast<const float4 *>(s.spheres), reinterpret_cast<const A *>(s.nsigmas),
                        reinterpret_cast<const A *>(s.texel_sites), reinterpret_cast<const A *>(s.texel_height),
-                       s.adjacency, s.adjacency_offsets, s.faces, (int)s.num_cells, s.num_texel_sites};
+                       s.adjacency, s.adjacency_offsets, s.faces, (int)s.num_cells, s.num_texel_sites,
+                       s.num_edges};
 }
 
 // Brent's cycle detection on the walk state.  The next cell is a function of
@@ -373,20 +420,21 @@ __device__ __forceinline__ RayFwd walk_ray_fwd(const SceneDev<A> &s, const RGB &
         const int e0 = s.off[prim];
         const int e1 = s.off[prim + 1];
         out.bytes += kVisitBytes + (uint64_t)(e1 - e0) * kFaceBytes;
-        int next = kNoCell;
+        int exit_e = -1;
         float pt_far = 1e10f;
-        for_faces<F>(s.faces, s.adj, e0, e1, [&](int, int nb, f3 fn, float fw) {
+        scan_row<F>(s.faces, s.E, e0, e1, [&](int e, f3 fn, float fw) {
             float t, dp;
             ray_pface_diff(o, d, fn, fw, t, dp);
-            if (dp >= 0.0f && t < pt_far) { next = nb; pt_far = t; }
+            if (dp >= 0.0f && t < pt_far) { exit_e = e; pt_far = t; }
             tf = dp >= 0.0f ? fmin_(t, tf) : tf;
             tn = dp < 0.0f ? fmax_(t, tn) : tn;
         });
+        const int next = exit_e >= 0 ? face_neighbour<F>(s.faces, s.adj, exit_e) : kNoCell;
         if (!sh.hit || tn > tf || sigma < 1e-3f) { prim = next; continue; }
 
         const int64_t tbase = (int64_t)prim * K;
         out.bytes += (uint64_t)K * kTexelBytes;
-        const TexFwd tx = texel_fwd<A, KT>(o, d, tn, c, nrm, r, s.sites + 3 * tbase, colors.cell(prim, K),
+        const TexFwdT<KT> tx = texel_fwd<A, KT>(o, d, tn, c, nrm, r, s.sites + 3 * tbase, colors.cell(prim, K),
                                           s.height + tbase, K);
         tf = tx.dp >= 0.0f ? fmin_(tx.t_surf, tf) : tf;
         tn = tx.dp < 0.0f ? fmax_(tx.t_surf, tn) : tn;
@@ -420,16 +468,14 @@ struct GradDev {
 // lds == nullptr adds straight into the global arrays; otherwise into the
 // cell's row of a wave's LDS accumulator (CellCache below), in the layout
 // sphere 0-3, nsigma 4-7, sites 8.., colours 8+3K.., heights 8+6K...
-// Zero contributions are skipped, as the reference's skipped texels add
-// nothing.
+// Zero contributions never reach global memory.
 struct CellSink {
     float *lds;
     float *gs, *gn, *gsite, *grgb, *gh;
     int K;
     __device__ __forceinline__ void add(int off, float *g, float v) const {
-        if (v == 0.0f) return;
-        if (lds) lds_add(lds + off, v);
-        else atomic_add(g, v);
+        if (lds) lds_add(lds + off, v);  // adding 0 to the row is harmless; flushes skip zeros
+        else if (v != 0.0f) atomic_add(g, v);
     }
     __device__ __forceinline__ void sphere(f3 dc, float dr) const {
         add(0, gs + 0, dc.x);
@@ -616,7 +662,7 @@ __device__ __forceinline__ int walk_ray_bwd(const SceneDev<A> &s, const RGB &col
         float r = 1.0f, sigma = 0.0f, tn = 0.0f, tf = 0.0f, tn0 = 0.0f, dt = 0.0f, e_dlt = 1.0f, alpha = 0.0f;
         int tn_id = -1, tf_id = -1, tn0_id = -1, tf0_id = -1, e0 = 0, n_adj = 0;
         int64_t tbase = 0;
-        TexFwd tx{};
+        TexFwdT<KT> tx{};
         if (live) {
             ++visits;
             const float4 sp = s.spheres[prim];
@@ -634,17 +680,18 @@ __device__ __forceinline__ int walk_ray_bwd(const SceneDev<A> &s, const RGB &col
             const int e1 = s.off[prim + 1];
             bytes += kVisitBytes + (uint64_t)(e1 - e0) * kFaceBytes;
             n_adj = e1 - e0;
-            int next = kNoCell;
+            int exit_e = -1;
             float pt_far = 1e10f;
-            for_faces<F>(s.faces, s.adj, e0, e1, [&](int e, int nb, f3 fn, float fw) {
+            scan_row<F>(s.faces, s.E, e0, e1, [&](int e, f3 fn, float fw) {
                 float t, dp;
                 ray_pface_diff(o, d, fn, fw, t, dp);
-                if (dp >= 0.0f && t < pt_far) { next = nb; pt_far = t; }
+                if (dp >= 0.0f && t < pt_far) { exit_e = e; pt_far = t; }
                 if (dp >= 0.0f && t < tf) tf_id = e - e0;
                 tf = dp >= 0.0f ? fmin_(t, tf) : tf;
                 if (dp < 0.0f && t > tn) tn_id = e - e0;
                 tn = dp < 0.0f ? fmax_(t, tn) : tn;
             });
+            const int next = exit_e >= 0 ? face_neighbour<F>(s.faces, s.adj, exit_e) : kNoCell;
             if (sh.hit && !(tn > tf) && !(sigma < 1e-3f)) {
                 tbase = (int64_t)prim * K;
                 bytes += (uint64_t)K * kTexelBytes;

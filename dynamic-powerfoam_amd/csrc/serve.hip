@@ -34,7 +34,7 @@ __device__ __forceinline__ int find_request(const pf_serve_request *reqs, int n,
 template <bool BWD, typename RGB>
 __device__ __forceinline__ int serve_ray(const SceneDev<float> &s, const RGB &colors, const pf_serve_request &rq,
                                          const Cam &cam, f3 d, int64_t pix, float thr, int max_steps) {
-    const RayFwd r = walk_ray_fwd<float, _Float16, 8>(s, colors, cam.eye, d, cam.start, thr, 0.5f, max_steps,
+    const RayFwd r = walk_ray_fwd<float, PackedF16, 8>(s, colors, cam.eye, d, cam.start, thr, 0.5f, max_steps,
                                                        nullptr, 0);
     rq.color[3 * pix] = r.rgb.x;
     rq.color[3 * pix + 1] = r.rgb.y;
@@ -47,13 +47,14 @@ __device__ __forceinline__ int serve_ray(const SceneDev<float> &s, const RGB &co
         const GradDev G{rq.grads.spheres, rq.grads.nsigmas, rq.grads.texel_sites, rq.grads.texel_rgb,
                         rq.grads.texel_height};
         uint64_t bytes = 0;
-        st |= walk_ray_bwd<float, _Float16, 8>(s, colors, cam.eye, d, cam.start, thr, max_steps, r.rgb, gc, 0.0f, G,
+        st |= walk_ray_bwd<float, PackedF16, 8>(s, colors, cam.eye, d, cam.start, thr, max_steps, r.rgb, gc, 0.0f, G,
                                                false, bytes);
     }
     return st;
 }
 
-// The table lists forward entries first: tiles [0, desc[2]) are forward
+// Served scenes use f32 attributes, K = 8 and packed walk faces
+// (PF_F16_PACKED).  The table lists forward entries first: tiles [0, desc[2]) are forward
 // only and [desc[2], desc[1]) forward+backward.  BWD selects the range, so
 // forward-only tiles run in a kernel without the backward's registers.
 template <int DEG, bool BWD, bool SH>

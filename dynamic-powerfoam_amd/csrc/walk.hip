@@ -75,12 +75,17 @@ __global__ __launch_bounds__(256) void walk_fwd_kernel(WalkParams p) {
     if (r.status && p.out.status) atomicOr(p.out.status, r.status);
 }
 
-// Slots of the per-wave gradient cache (K = 8: 64-float rows, 12.5 KB of
-// LDS per wave, 50 KB per block, three blocks per CU).
-constexpr int kCacheSlots = 48;
+// Slots of the per-wave gradient cache (K = 8: 65-float rows, 9.9 KB of
+// LDS per wave, 39.5 KB per block): four blocks per CU, which matches the
+// kernel's register budget of 128 VGPRs (four waves per SIMD).  48 slots
+// catch more reuse but hold three blocks per CU and measured slower.
+constexpr int kCacheSlots = 38;
 
+// Four waves per SIMD (the LDS cache allows four blocks per CU): the
+// register budget is capped at 128, which costs a few spilled registers and
+// measured faster than three waves with no spills.
 template <typename A, typename F, int KT>
-__global__ __launch_bounds__(256) void walk_bwd_kernel(WalkParams p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void walk_bwd_kernel(WalkParams p) {
     const TileLane tl = tile_lane(p, 4);
     constexpr bool kWave = KT == 8;
     if (!kWave && !tl.valid) return;
@@ -115,10 +120,32 @@ __global__ __launch_bounds__(256) void walk_bwd_kernel(WalkParams p) {
     if (st && p.status) atomicOr(p.status, st);
 }
 
+// Stores one walk face; the packed layout carries the neighbour too.
+template <typename F> __device__ __forceinline__ void store_face(void *faces, int64_t e, f3 dp, float dpm, int) {
+    F *f = reinterpret_cast<F *>(faces) + 4 * e;
+    f[0] = (F)dp.x;
+    f[1] = (F)dp.y;
+    f[2] = (F)dp.z;
+    f[3] = (F)dpm;
+}
+template <> __device__ __forceinline__ void store_face<PackedF16>(void *faces, int64_t e, f3 dp, float dpm, int nb) {
+    reinterpret_cast<PackedFace *>(faces)[e] = PackedFace{(_Float16)dp.x, (_Float16)dp.y, (_Float16)dp.z,
+                                                          (_Float16)dpm, nb};
+}
+
+// Reference-layout faces (E,4) f16 plus the adjacency into packed records.
+__global__ __launch_bounds__(256) void pack_faces_kernel(const h4 *faces, const int32_t *adj, int64_t E,
+                                                         PackedFace *out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const h4 f = faces[e];
+    out[e] = PackedFace{f.x, f.y, f.z, f.w, adj[e]};
+}
+
 // Walk faces: benchmark.py:295-302, one thread per cell over its CSR row.
 template <typename F>
 __global__ void walk_faces_kernel(const float *points, const float *radii, int64_t N, const int32_t *adj,
-                                  const int32_t *off, F *faces) {
+                                  const int32_t *off, void *faces) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const f3 pi = mk(points[3 * i], points[3 * i + 1], points[3 * i + 2]);
@@ -131,10 +158,7 @@ __global__ void walk_faces_kernel(const float *points, const float *radii, int64
         const float rj = radii[j];
         const float nj = __builtin_sqrtf(dot(pj, pj));
         const float pmj = 0.5f * (nj * nj - rj * rj);
-        faces[4 * (int64_t)e + 0] = (F)(pj.x - pi.x);
-        faces[4 * (int64_t)e + 1] = (F)(pj.y - pi.y);
-        faces[4 * (int64_t)e + 2] = (F)(pj.z - pi.z);
-        faces[4 * (int64_t)e + 3] = (F)(pmj - pmi);
+        store_face<F>(faces, e, mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), pmj - pmi, j);
     }
 }
 
@@ -203,7 +227,10 @@ static void launch_walk(const WalkParams &p, bool bwd, int K, hipStream_t st) {
 
 static int dispatch_walk(const WalkParams &p, bool bwd, hipStream_t st) {
     const int K = p.s.num_texel_sites;
-    if (p.s.attr_dtype == PF_F32 && p.s.face_dtype == PF_F16) launch_walk<float, _Float16>(p, bwd, K, st);
+    if (p.s.attr_dtype == PF_F32 && p.s.face_dtype == PF_F16_PACKED) launch_walk<float, PackedF16>(p, bwd, K, st);
+    else if (p.s.attr_dtype == PF_F16 && p.s.face_dtype == PF_F16_PACKED)
+        launch_walk<_Float16, PackedF16>(p, bwd, K, st);
+    else if (p.s.attr_dtype == PF_F32 && p.s.face_dtype == PF_F16) launch_walk<float, _Float16>(p, bwd, K, st);
     else if (p.s.attr_dtype == PF_F16 && p.s.face_dtype == PF_F16) launch_walk<_Float16, _Float16>(p, bwd, K, st);
     else if (p.s.attr_dtype == PF_F32 && p.s.face_dtype == PF_F32) launch_walk<float, float>(p, bwd, K, st);
     else if (p.s.attr_dtype == PF_F16 && p.s.face_dtype == PF_F32) launch_walk<_Float16, float>(p, bwd, K, st);
@@ -217,7 +244,8 @@ static int check_scene(const pf_scene *s) {
     if (s->num_edges > 0 && (!s->adjacency || !s->faces)) return PF_ERR_ARG;
     if (s->num_cells <= 0 || s->num_cells >= 0x7FFFFFFF) return PF_ERR_ARG;
     if (s->num_texel_sites < 1 || s->num_texel_sites > 16) return PF_ERR_ARG;
-    if ((s->attr_dtype != PF_F32 && s->attr_dtype != PF_F16) || (s->face_dtype != PF_F32 && s->face_dtype != PF_F16))
+    if ((s->attr_dtype != PF_F32 && s->attr_dtype != PF_F16) ||
+        (s->face_dtype != PF_F32 && s->face_dtype != PF_F16 && s->face_dtype != PF_F16_PACKED))
         return PF_ERR_DTYPE;
     return PF_OK;
 }
@@ -283,11 +311,23 @@ extern "C" int pf_walk_faces(const float *points, const float *radii, int64_t nu
     hipStream_t st = (hipStream_t)stream;
     if (face_dtype == PF_F16)
         hipLaunchKernelGGL((walk_faces_kernel<_Float16>), dim3(blocks), dim3(256), 0, st, points, radii, num_cells,
-                           adjacency, adjacency_offsets, (_Float16 *)faces);
+                           adjacency, adjacency_offsets, faces);
     else if (face_dtype == PF_F32)
         hipLaunchKernelGGL((walk_faces_kernel<float>), dim3(blocks), dim3(256), 0, st, points, radii, num_cells,
-                           adjacency, adjacency_offsets, (float *)faces);
+                           adjacency, adjacency_offsets, faces);
+    else if (face_dtype == PF_F16_PACKED)
+        hipLaunchKernelGGL((walk_faces_kernel<PackedF16>), dim3(blocks), dim3(256), 0, st, points, radii, num_cells,
+                           adjacency, adjacency_offsets, faces);
     else return PF_ERR_DTYPE;
+    return hipGetLastError() == hipSuccess ? PF_OK : PF_ERR_HIP;
+}
+
+extern "C" int pf_walk_pack_faces(const void *faces_f16, const int32_t *adjacency, int64_t num_edges, void *packed,
+                                  void *stream) {
+    if (num_edges < 0 || (num_edges > 0 && (!faces_f16 || !adjacency || !packed))) return PF_ERR_ARG;
+    if (num_edges == 0) return PF_OK;
+    hipLaunchKernelGGL(pack_faces_kernel, dim3((unsigned)((num_edges + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, (const h4 *)faces_f16, adjacency, num_edges, (PackedFace *)packed);
     return hipGetLastError() == hipSuccess ? PF_OK : PF_ERR_HIP;
 }
 

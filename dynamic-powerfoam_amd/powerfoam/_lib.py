@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libpf_hip.so")
 
 PF_OK, PF_ERR_ARG, PF_ERR_DTYPE, PF_ERR_HIP, PF_ERR_OVERFLOW = 0, 1, 2, 3, 4
 PF_STATUS_STEP_CAP, PF_STATUS_LIST_CAP, PF_STATUS_CYCLE = 1, 2, 4
-PF_F32, PF_F16 = 0, 1
+PF_F32, PF_F16, PF_F16_PACKED = 0, 1, 2
 PF_BWD_EXACT_TEXEL = 1
 PF_SERVE_SH_IN_WALK = 1
 
@@ -136,6 +136,7 @@ _SIGS = {
         [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p, c_int32, c_void_p],
     ),
     "pf_start_cells": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_int32, c_void_p, c_void_p]),
+    "pf_walk_pack_faces": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "pf_raster_faces": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int32, c_void_p]),
     "pf_raster_bin_count": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
     "pf_raster_scan": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p]),

@@ -34,7 +34,7 @@ class WalkScene:
     """
 
     def __init__(self, points, radii, density, normals, texel_sites, texel_rgb, texel_height, adjacency,
-                 adjacency_offsets, faces, attr_dtype=torch.float32):
+                 adjacency_offsets, faces, attr_dtype=torch.float32, pack_faces=True):
         dev = points.device
         if dev.type != "cuda":
             raise ValueError("Only CUDA tensors are supported as inputs")
@@ -56,8 +56,17 @@ class WalkScene:
         if self.offsets.numel() != n + 1:
             raise ValueError("adjacency_offsets must have N+1 entries")
         self.faces = _contig(faces)
-        if self.faces.dtype not in (torch.float16, torch.float32) or self.faces.shape != (self.adjacency.numel(), 4):
-            raise ValueError("adjacency_diff must be (E, 4) float16 or float32")
+        E = self.adjacency.numel()
+        if self.faces.dtype == torch.int32 and self.faces.shape == (E, 3):
+            face_code = _lib.PF_F16_PACKED  # already packed records
+        elif self.faces.dtype in (torch.float16, torch.float32) and self.faces.shape == (E, 4):
+            face_code = _lib.dtype_code(self.faces.dtype)
+            if pack_faces and face_code == _lib.PF_F16:
+                # same values, one 12-byte record per face: fewer gathers
+                self.faces = pack_walk_faces(self.faces, self.adjacency)
+                face_code = _lib.PF_F16_PACKED
+        else:
+            raise ValueError("adjacency_diff must be (E, 4) float16 or float32, or (E, 3) int32 packed records")
         s = _lib.PfScene()
         s.spheres = _lib.ptr(self.spheres)
         s.nsigmas = _lib.ptr(self.nsigmas)
@@ -71,7 +80,7 @@ class WalkScene:
         s.num_edges = self.adjacency.numel()
         s.num_texel_sites = self.K
         s.attr_dtype = _lib.dtype_code(attr_dtype)
-        s.face_dtype = _lib.dtype_code(self.faces.dtype)
+        s.face_dtype = face_code
         self.c = s
 
 
@@ -87,6 +96,20 @@ def start_cells_device(points, radii, cams_dev):
                             _lib.stream_ptr())
     _lib.check(rc, "pf_start_cells")
     return cams_dev
+
+
+def pack_walk_faces(faces_f16, adjacency):
+    """(E, 4) f16 walk faces + adjacency -> (E, 3) int32 packed records
+    (PF_F16_PACKED): the same fp16 values with the neighbour alongside."""
+    lib = _lib.load()
+    f = _contig(faces_f16)
+    adj = _contig(adjacency, torch.int32)
+    if f.dtype != torch.float16 or f.shape != (adj.numel(), 4):
+        raise ValueError("faces must be (E, 4) float16")
+    out = torch.empty(adj.numel(), 3, dtype=torch.int32, device=f.device)
+    _lib.check(lib.pf_walk_pack_faces(_lib.ptr(f), _lib.ptr(adj), adj.numel(), _lib.ptr(out), _lib.stream_ptr()),
+               "pf_walk_pack_faces")
+    return out
 
 
 def walk_faces(points, radii, adjacency, adjacency_offsets, dtype=torch.float16):

@@ -82,7 +82,7 @@ class RenderRequest:
 
 
 class ResidentScene:
-    """Device arrays of one scene (f32 attributes, f16 walk faces, K = 8)
+    """Device arrays of one scene (f32 attributes, packed f16 walk faces, K = 8)
     plus optional degree-3 SH coefficients (turned into texel colours by a
     per-request pre-pass, or evaluated inside the walk)."""
 
@@ -91,13 +91,14 @@ class ResidentScene:
         if sh is None and texel_rgb is None:
             raise ValueError("a scene needs SH coefficients or texel_rgb")
         K = texel_sites.shape[1]
-        if K != 8 or faces.dtype != torch.float16:
-            raise ValueError("served scenes use K = 8 texels and fp16 walk faces")
+        if K != 8 or faces.dtype not in (torch.float16, torch.int32):
+            raise ValueError("served scenes use K = 8 texels and fp16 (or packed) walk faces")
         # with SH colour the kernel never reads texel_rgb; texel_sites (same
         # shape) stands in so the scene struct holds a valid pointer
         rgb = texel_rgb if texel_rgb is not None else texel_sites
         self.walk = WalkScene(points, radii, density, normals, texel_sites, rgb, texel_height, adjacency,
                               adjacency_offsets, faces)
+        assert self.walk.c.face_dtype == _lib.PF_F16_PACKED  # the step kernel reads packed faces
         self.points = points.detach().float().contiguous()
         self.radii = radii.detach().float().contiguous()
         self.sh = sh.detach().float().contiguous() if sh is not None else None

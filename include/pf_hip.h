@@ -40,6 +40,11 @@ extern "C" {
 /* ---- dtypes ------------------------------------------------------------ */
 #define PF_F32 0
 #define PF_F16 1
+#define PF_F16_PACKED 2   /* walk faces only: (E,3) 32-bit words per edge,
+                             [half dp.x | half dp.y], [half dp.z | half dpm],
+                             [int32 neighbour]; one 12-byte record per face,
+                             so a face and its neighbour arrive in one load
+                             (the values are those of PF_F16)             */
 
 /* Pinhole camera, the fields of WarpCamera used on the pinhole path
  * (powerfoam/camera.py:8-17).  start_cell is the walk's first cell
@@ -70,7 +75,7 @@ typedef struct pf_scene {
     int64_t num_edges;
     int32_t num_texel_sites;          /* K, 1..16                           */
     int32_t attr_dtype;               /* PF_F32 | PF_F16                    */
-    int32_t face_dtype;               /* PF_F32 | PF_F16                    */
+    int32_t face_dtype;               /* PF_F32 | PF_F16 | PF_F16_PACKED    */
     int32_t reserved;
     int64_t texel_rgb_cam_stride;     /* elements between cameras' texel_rgb
                                          (view-dependent colour); 0: shared */
@@ -152,11 +157,17 @@ int pf_walk_backward(const pf_scene *scene, const pf_camera *cams,
 
 /* Walk face precompute (benchmark.py:295-302): for every CSR edge i->j,
  * faces[e] = half(p_j - p_i, pm_j - pm_i) with pm = 0.5(|p|^2 - r^2).
- * face_dtype PF_F16 (reference) or PF_F32.                                 */
+ * face_dtype PF_F16 (reference), PF_F32 or PF_F16_PACKED.                  */
 int pf_walk_faces(const float *points, const float *radii, int64_t num_cells,
                   const int32_t *adjacency, const int32_t *adjacency_offsets,
                   int64_t num_edges, void *faces, int32_t face_dtype,
                   void *stream);
+
+/* Repacks reference-layout walk faces ((E,4) f16, the adjacency_diff that
+ * benchmark.py:295-302 builds) and the adjacency into PF_F16_PACKED
+ * records (E,3) int32.  No reference counterpart (a layout change only).  */
+int pf_walk_pack_faces(const void *faces_f16, const int32_t *adjacency,
+                       int64_t num_edges, void *packed, void *stream);
 
 /* Start cell (benchmark.py:320-324): for each camera,
  * argmin_i |p_i - eye|^2 - r_i^2 (first index on ties), written into
@@ -332,6 +343,8 @@ int pf_cech_write(const float *centers, const float *radii, int64_t num_cells,
 /* A scene resident on the device: f32 attributes, f16 walk faces, K = 8.
  * sh: (N,16,3) degree-3 coefficients evaluated per texel site inside the
  * walk, or NULL to use scene.texel_rgb.                                    */
+/* A served scene: f32 attributes, K = 8 texels, PF_F16_PACKED walk faces;
+ * sh (N,16,3) f32 degree-3 coefficients or NULL (then texel_rgb).        */
 typedef struct pf_serve_scene {
     pf_scene scene;
     const float *sh;

@@ -10,6 +10,7 @@ import torch
 
 from oracle import fp64
 from oracle import oracle as orc
+from powerfoam import _lib
 from powerfoam import raytrace as rt
 from powerfoam import workloads
 from powerfoam.camera import cameras_to_device, pack_cameras
@@ -19,13 +20,13 @@ TOL = 1e-4
 DEV = "cuda"
 
 
-def _gpu_scene(sc, attr=torch.float32, face_f16=True):
+def _gpu_scene(sc, attr=torch.float32, face_f16=True, packed=True):
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
     pts, rad = t(sc["points"]), t(sc["radii"])
     adj, off = t(sc["adjacency"]).int(), t(sc["offsets"]).int()
     faces = rt.walk_faces(pts, rad, adj, off, torch.float16 if face_f16 else torch.float32)
     s = rt.WalkScene(pts, rad, t(sc["density"]), t(sc["normals"]), t(sc["texel_sites"]), t(sc["texel_rgb"]),
-                     t(sc["texel_height"]), adj, off, faces, attr_dtype=attr)
+                     t(sc["texel_height"]), adj, off, faces, attr_dtype=attr, pack_faces=packed)
     return s, faces
 
 
@@ -45,8 +46,8 @@ def _cams(sc, cams):
     return pack_cameras(cams, starts)
 
 
-def _compare_forward(sc, cams, attr=torch.float32, face_f16=True, seq_cap=64):
-    gs, faces = _gpu_scene(sc, attr, face_f16)
+def _compare_forward(sc, cams, attr=torch.float32, face_f16=True, seq_cap=64, packed=True):
+    gs, faces = _gpu_scene(sc, attr, face_f16, packed)
     words = _cams(sc, cams)
     cams_dev = torch.from_numpy(words).to(DEV)
     W, H = cams[0].width, cams[0].height
@@ -92,9 +93,34 @@ def test_walk_uniform_4k_128px_bit_exact(scene4k):
     assert np.array_equal(g["color"].cpu().numpy(), c["color"])
 
 
-@pytest.mark.parametrize("attr,f16", [(torch.float16, True), (torch.float32, False), (torch.float16, False)])
-def test_walk_dtypes(scene4k, attr, f16):
-    _compare_forward(scene4k, [workloads.front_camera(64, 48)], attr=attr, face_f16=f16)
+@pytest.mark.parametrize("attr,f16,packed", [(torch.float16, True, True), (torch.float32, False, False),
+                                             (torch.float16, False, False), (torch.float32, True, False),
+                                             (torch.float16, True, False)])
+def test_walk_dtypes(scene4k, attr, f16, packed):
+    """Attribute dtypes x face layouts (fp16 packed records, fp16, fp32)."""
+    gs, _ = _gpu_scene(scene4k, attr, f16, packed)
+    want = _lib.PF_F16_PACKED if (f16 and packed) else (_lib.PF_F16 if f16 else _lib.PF_F32)
+    assert gs.c.face_dtype == want
+    _compare_forward(scene4k, [workloads.front_camera(64, 48)], attr=attr, face_f16=f16, packed=packed)
+
+
+def test_packed_faces_match_reference_layout(scene4k):
+    """Packing reference-layout fp16 faces gives the records pf_walk_faces
+    writes directly (same fp16 bits, neighbour alongside)."""
+    sc = scene4k
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+    pts, rad = t(sc["points"]), t(sc["radii"])
+    adj, off = t(sc["adjacency"]).int(), t(sc["offsets"]).int()
+    f16 = rt.walk_faces(pts, rad, adj, off, torch.float16)
+    packed = rt.pack_walk_faces(f16, adj)
+    direct = torch.empty_like(packed)
+    rc = _lib.load().pf_walk_faces(_lib.ptr(pts), _lib.ptr(rad), pts.shape[0], _lib.ptr(adj), _lib.ptr(off),
+                                   adj.numel(), _lib.ptr(direct), _lib.PF_F16_PACKED, _lib.stream_ptr())
+    _lib.check(rc, "pf_walk_faces")
+    torch.cuda.synchronize()
+    assert torch.equal(packed, direct)
+    assert torch.equal(packed[:, 2], adj)
+    assert torch.equal(packed[:, :2].contiguous().view(torch.float16).view(-1, 4), f16)
 
 
 def test_walk_batch_ragged_tiles(scene4k):
