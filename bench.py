@@ -43,6 +43,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=160, help="requests in the serving trace (0 = skip)")
     ap.add_argument("--serve-load", type=float, default=0.6, help="offered load as a fraction of plain capacity")
     ap.add_argument("--serve-capacity", type=int, default=16384, help="8x8 tiles per serving step")
+    ap.add_argument("--adjacency", choices=("gpu", "qhull"), default="gpu",
+                    help="power-diagram adjacency from the GPU builder or the CPU Qhull lift (same edges)")
     ap.add_argument("--serve-quantum", type=int, default=1024, help="tiles per request per step before leftovers")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (reference kernel for HBM byte counters)")
@@ -70,11 +72,18 @@ def main():
     sc = workloads.gmm_scene(args.cells, seed=args.seed)
     cam = workloads.sphere_cameras(sc["rng"], 1, args.width, args.height)[0]
     target_np = sc["rng"].uniform(0.0, 1.0, (1, args.height, args.width, 3)).astype(np.float32)
-    log(f"scene {args.cells} cells generated in {time.time() - t0:.1f}s; building power adjacency (Qhull)")
+    log(f"scene {args.cells} cells generated in {time.time() - t0:.1f}s; building power adjacency ({args.adjacency})")
     t1 = time.time()
-    adj, off = build_power_adjacency(torch.from_numpy(sc["points"]), torch.from_numpy(sc["radii"]))
-    log(f"adjacency: {adj.numel()} directed edges (mean degree {adj.numel() / args.cells:.2f}) "
-        f"in {time.time() - t1:.1f}s")
+    if args.adjacency == "gpu":
+        from powerfoam.power import build_power_adjacency_gpu
+
+        adj, off = build_power_adjacency_gpu(torch.from_numpy(sc["points"]).to(dev), torch.from_numpy(sc["radii"]).to(dev))
+        torch.cuda.synchronize()
+        adj, off = adj.cpu(), off.cpu()
+    else:
+        adj, off = build_power_adjacency(torch.from_numpy(sc["points"]), torch.from_numpy(sc["radii"]))
+    adj_seconds = time.time() - t1
+    log(f"adjacency: {adj.numel()} directed edges (mean degree {adj.numel() / args.cells:.2f}) in {adj_seconds:.1f}s")
 
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
     pts, rad = t(sc["points"]), t(sc["radii"])
@@ -187,6 +196,8 @@ def main():
             "height": H,
             "rays_per_step": rays,
             "face_storage": "f16",
+            "adjacency": args.adjacency,
+            "adjacency_build_s": round(adj_seconds, 2),
             "attribute_storage": "f32",
             "mean_visits_per_ray": round(visits.mean().item(), 2),
             "walk_fwd_ms": round(fwd_ms, 4),

@@ -176,6 +176,16 @@ _SIGS = {
         [c_void_p, c_void_p, c_int64, ctypes.POINTER(PfCechGrid), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
          c_void_p],
     ),
+    "pf_power_occupancy_bytes": (c_int64, [ctypes.POINTER(PfCechGrid)]),
+    "pf_power_cells": (
+        c_int32,
+        [c_void_p, c_void_p, c_int64, ctypes.POINTER(PfCechGrid), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+         c_void_p, c_int64, c_void_p, c_float, c_float, c_int32, c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
+    "pf_power_hull": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_float, c_int32, c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
     "pf_serve_step": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_float, c_int32, c_int32, c_void_p, c_void_p]),
     "pf_sh_forward": (
         c_int32,

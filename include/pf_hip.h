@@ -393,6 +393,47 @@ int pf_serve_step(const pf_serve_scene *scenes,
                   int32_t max_steps, int32_t flags, int32_t *status,
                   void *stream);
 
+/* ---- power-diagram adjacency ----------------------------------------- */
+
+/* The power diagram's adjacency (build_power_adjacency, benchmark.py:25-83,
+ * which lifts the sites to 4-D for a Qhull hull) for the part of the
+ * diagram inside box = {lo[3], hi[3]}: for each site i, the sites j whose
+ * bisector face with i meets the box.  One thread per site clips the box by
+ * the bisectors of its nearest sites in FP64, then certifies each vertex of
+ * the cell with an empty-ball lookup in an occupancy pyramid over the grid
+ * buckets.  grid (cell size and dims over the sites' bounds, dims <= 2^20
+ * per axis) is chosen by the caller; grid_counts (G+1) and grid_fill (G)
+ * zeroed by the caller, cell_key and grid_items N, grid_start G+1;
+ * occupancy: pf_power_occupancy_bytes(grid) bytes (any contents).
+ * lists (N, max_nb) int32 (rows sorted ascending), counts (N), flags (N):
+ * bit 0 the cell meets the box boundary (its faces outside the box are
+ * those of the diagram of the flagged sites alone), bit 1 capacity
+ * overflow (max_nb <= 64; the row is incomplete), bit 2 hidden site
+ * (empty cell).  lifted (N): the lift L = |p|^2 - r^2 as the caller
+ * computed it (the reference lifts in f32, benchmark.py:29-32); the
+ * bisector of i and j is 2 x.(p_j - p_i) <= L_j - L_i.  r_max: largest
+ * radius; lift_slack >= |L - (|p|^2 - r^2)| (the lift's rounding).       */
+int64_t pf_power_occupancy_bytes(const pf_cech_grid *grid);
+int pf_power_cells(const float *points, const float *lifted, int64_t num_cells,
+                   const pf_cech_grid *grid, int32_t *grid_counts,
+                   int32_t *grid_fill, int32_t *cell_key, int64_t *grid_start,
+                   int32_t *grid_items, uint8_t *occupancy,
+                   int64_t occupancy_bytes, const float *box, float r_max,
+                   float lift_slack, int32_t max_nb, int32_t *lists,
+                   int32_t *counts, int32_t *flags, void *stream);
+
+/* The faces of the power diagram outside box: for each of the num_hull
+ * sites whose cell meets the box boundary (flag bit 0 of pf_power_cells),
+ * the other such sites j whose bisector face with it has a vertex outside
+ * the box.  Outside the box only those sites own points, so this completes
+ * pf_power_cells' adjacency.  Faces farther than far from a site are not
+ * seen.  lists (num_hull, max_nb) of site ids, counts and flags as
+ * pf_power_cells.                                                        */
+int pf_power_hull(const float *points, const float *lifted, const int32_t *hull,
+                  int32_t num_hull, const float *box, float far,
+                  int32_t max_nb, int32_t *lists, int32_t *counts,
+                  int32_t *flags, void *stream);
+
 /* ---- colour pre-pass ---------------------------------------------------- */
 
 /* Spherical-harmonics colour per texel site and camera (degree 0..3,
