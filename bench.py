@@ -43,6 +43,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=160, help="requests in the serving trace (0 = skip)")
     ap.add_argument("--serve-load", type=float, default=0.6, help="offered load as a fraction of plain capacity")
     ap.add_argument("--serve-capacity", type=int, default=16384, help="8x8 tiles per serving step")
+    ap.add_argument("--cell-order", choices=("morton", "input"), default="morton",
+                    help="cells in Morton order (as the reference's training leaves them) or as generated")
     ap.add_argument("--adjacency", choices=("gpu", "qhull"), default="gpu",
                     help="power-diagram adjacency from the GPU builder or the CPU Qhull lift (same edges)")
     ap.add_argument("--serve-quantum", type=int, default=1024, help="tiles per request per step before leftovers")
@@ -72,6 +74,14 @@ def main():
     sc = workloads.gmm_scene(args.cells, seed=args.seed)
     cam = workloads.sphere_cameras(sc["rng"], 1, args.width, args.height)[0]
     target_np = sc["rng"].uniform(0.0, 1.0, (1, args.height, args.width, 3)).astype(np.float32)
+    if args.cell_order == "morton":
+        # sort_points (scene.py:197-233) leaves a trained scene in this order
+        from powerfoam.geometry import morton_sort
+
+        perm = morton_sort(torch.from_numpy(sc["points"]).to(dev)).long().cpu().numpy()
+        for k in ("points", "radii", "density", "normals", "texel_sites", "texel_rgb", "texel_height", "sh"):
+            if sc[k] is not None:
+                sc[k] = np.ascontiguousarray(sc[k][perm])
     log(f"scene {args.cells} cells generated in {time.time() - t0:.1f}s; building power adjacency ({args.adjacency})")
     t1 = time.time()
     if args.adjacency == "gpu":
@@ -196,6 +206,7 @@ def main():
             "height": H,
             "rays_per_step": rays,
             "face_storage": "f16",
+            "cell_order": args.cell_order,
             "adjacency": args.adjacency,
             "adjacency_build_s": round(adj_seconds, 2),
             "attribute_storage": "f32",

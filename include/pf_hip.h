@@ -434,6 +434,18 @@ int pf_power_hull(const float *points, const float *lifted, const int32_t *hull,
                   int32_t max_nb, int32_t *lists, int32_t *counts,
                   int32_t *flags, void *stream);
 
+/* ---- cell order ---------------------------------------------------------- */
+
+/* 63-bit Morton codes of points (N,3) f32 over the box [lower, upper]
+ * (host float[3] each): per axis q = (p - lower) / max(upper - lower, 1e-6)
+ * * 2^21, clamped to [0, 2^21 - 1] and truncated; the bits of x, y, z
+ * interleaved with x highest.  Replaces morton_code_kernel (geometry.py:
+ * 18-47); sorting the codes stably gives morton_sort (geometry.py:50-85),
+ * the order sort_points (scene.py:197-233) puts cells in.  codes (N) int64. */
+int pf_morton_codes(const float *points, int64_t num_points,
+                    const float *lower, const float *upper, int64_t *codes,
+                    void *stream);
+
 /* ---- colour pre-pass ---------------------------------------------------- */
 
 /* Spherical-harmonics colour per texel site and camera (degree 0..3,

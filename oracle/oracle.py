@@ -389,3 +389,26 @@ def sv_backward(sites, eye, fwd, cutoff, axes, vals, temps, wsum, value, grad, f
     fn(_p(sites), n, _p(eye), _p(fwd), cutoff, _p(axes), _p(vals), _p(temps), dof, _p(_c(wsum, dt)),
        _p(_c(value, dt)), _p(_c(grad, np.float32)), _p(ga), _p(gv), _p(gt))
     return ga, gv, gt
+
+
+def morton_codes(points):
+    """Morton codes as morton_code_kernel (geometry.py:18-47) computes them:
+    f32 normalisation over the bounding box, 2^21 steps per axis clamped and
+    truncated, bits interleaved with x highest (expand_bits, geometry.py:7-15).
+    int64 (N,)."""
+    p = np.asarray(points, dtype=np.float32)
+    if p.shape[0] == 0:
+        return np.zeros(0, np.int64)
+    lo, hi = p.min(0), p.max(0)
+    ext = np.maximum(hi - lo, np.float32(1e-6))
+    res = np.float32(2097152.0)
+    q = np.clip((p - lo) / ext * res, np.float32(0.0), res - np.float32(1.0)).astype(np.uint64)
+
+    def spread(v):
+        v = v & np.uint64(0x1FFFFF)
+        for shift, mask in ((32, 0x1F00000000FFFF), (16, 0x1F0000FF0000FF), (8, 0x100F00F00F00F00F),
+                            (4, 0x10C30C30C30C30C3), (2, 0x1249249249249249)):
+            v = (v | (v << np.uint64(shift))) & np.uint64(mask)
+        return v
+
+    return ((spread(q[:, 0]) << np.uint64(2)) | (spread(q[:, 1]) << np.uint64(1)) | spread(q[:, 2])).astype(np.int64)
