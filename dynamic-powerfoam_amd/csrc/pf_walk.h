@@ -69,27 +69,27 @@ template <> struct FaceT<PackedF16> {
 // fn(e, normal, weight).  The neighbour across a face is not fetched here:
 // the walk needs it for the exit face only (face_neighbour after the scan),
 // and every load saved counts, since a wave's lanes gather from 64
-// different cells.  Faces are fetched kFaceBatch at a time before any is
-// used; the tail of the last batch re-reads the row's last face (in
-// bounds) and is not passed to fn.
+// different cells.  Faces are fetched B at a time before any is used.  The
+// tail of the last batch re-reads the row's last face and passes it to fn
+// again, with its own index: every update the walk makes from a face is a
+// strict comparison or a min / max, so a repeated face changes nothing,
+// and the batch needs no branch per face.
 #ifndef PF_FACE_BATCH
 #define PF_FACE_BATCH 4
 #endif
 constexpr int kFaceBatch = PF_FACE_BATCH;
-template <typename F, typename Fn>
+template <typename F, int B = kFaceBatch, typename Fn>
 __device__ __forceinline__ void for_faces(const void *faces, int64_t, int e0, int e1, Fn &&fn) {
-    for (int e = e0; e < e1; e += kFaceBatch) {
-        typename FaceT<F>::Raw raw[kFaceBatch];
+    for (int e = e0; e < e1; e += B) {
+        typename FaceT<F>::Raw raw[B];
 #pragma unroll
-        for (int k = 0; k < kFaceBatch; ++k) raw[k] = FaceT<F>::raw(faces, e + k < e1 ? e + k : e1 - 1);
+        for (int k = 0; k < B; ++k) raw[k] = FaceT<F>::raw(faces, e + k < e1 ? e + k : e1 - 1);
 #pragma unroll
-        for (int k = 0; k < kFaceBatch; ++k) {
-            if (e + k < e1) {
-                f3 n;
-                float w;
-                FaceT<F>::decode(raw[k], n, w);
-                fn(e + k, n, w);
-            }
+        for (int k = 0; k < B; ++k) {
+            f3 n;
+            float w;
+            FaceT<F>::decode(raw[k], n, w);
+            fn(e + k < e1 ? e + k : e1 - 1, n, w);
         }
     }
 }
@@ -102,9 +102,9 @@ template <> __device__ __forceinline__ int face_neighbour<PackedF16>(const void 
     return reinterpret_cast<const PackedFace *>(faces)[e].nb;
 }
 
-template <typename F, typename Fn>
+template <typename F, int B = kFaceBatch, typename Fn>
 __device__ __forceinline__ void scan_row(const void *faces, int64_t E, int e0, int e1, Fn &&fn) {
-    for_faces<F>(faces, E, e0, e1, fn);
+    for_faces<F, B>(faces, E, e0, e1, fn);
 }
 
 template <typename A> __device__ __forceinline__ f3 ld3(const A *p) {
@@ -120,6 +120,7 @@ __device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(
 // LDS access through an explicitly LDS-qualified pointer, so that it
 // compiles to ds_* instructions rather than flat ones.
 typedef __attribute__((address_space(3))) float lds_float;
+typedef __attribute__((address_space(3))) int lds_int;
 __device__ __forceinline__ lds_float *as_lds(float *p) { return (lds_float *)p; }
 __device__ __forceinline__ void lds_add(float *p, float v) {
     __hip_atomic_fetch_add(as_lds(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -612,6 +613,92 @@ template <int S, int KT> struct CellCache {
     }
 };
 
+// One contributing segment of a replayed ray: what its gradients need
+// besides the ray (o, d), its incoming gradients and the texel pass.
+struct Seg {
+    int cell;
+    f3 c, nrm;
+    float r, sigma, tn0, dt, e_dlt, alpha, trans;
+    f3 rgb;  // the ray's colour after this segment
+    int tn_id, tf_id, tn0_id, tf0_id, e0, n_adj;
+    int64_t tbase;
+};
+
+// Algorithmic bytes of one segment's gradients: the read-modify-write of
+// its cell's row, and 32 per bounding neighbour face.
+__device__ __forceinline__ uint64_t segment_grad_bytes(int K, int tn0_id, int tf0_id, int n_adj) {
+    return 2ull * (32ull + (uint64_t)K * 28ull) + (tn0_id >= 0 && tn0_id < n_adj ? 32ull : 0ull) +
+           (tf0_id >= 0 && tf0_id < n_adj ? 32ull : 0ull);
+}
+
+// Gradients of one segment (rasterize.py:1371-1516, texture.py:74-186):
+//   dL/d(delta_s) = g.(C - C_{<=s}) + g_logT - (g.c_s) T_s exp(delta_s),
+// routed to the sphere, a power face or the texel plane.  The cell's own
+// gradients go to row (an LDS row of the wave's cache) or, row == nullptr,
+// to global memory; neighbour spheres get global atomics.
+template <typename A, typename F, int KT, typename RGB>
+__device__ __forceinline__ void segment_grads(const SceneDev<A> &s, const RGB &colors, f3 o, f3 d, const Seg &q,
+                                              const TexFwdT<KT> &tx, f3 g, float gL, f3 Cfin, const GradDev &G,
+                                              bool exact, float *row) {
+    const int K = KT > 0 ? KT : s.K;
+    const int cell = q.cell, n_adj = q.n_adj;
+    const f3 c = q.c;
+    const float r = q.r, sigma = q.sigma, trans = q.trans, dt = q.dt;
+    const float dL_dLnext = dot(g, Cfin - q.rgb) + gL;
+    const float dL_dalpha = dot(g, tx.rgb) * trans;
+    const f3 dL_dcolor = (g * trans) * q.alpha;
+    const float dL_ddelta = dL_dLnext + (-dL_dalpha * q.e_dlt);
+    const float dL_dsigma = dL_ddelta * (-dt);
+    const float dL_dtn = dL_ddelta * sigma;
+    const float dL_dtf = dL_ddelta * (-sigma);
+
+    float dL_dts = 0.0f;
+    if (q.tn_id == n_adj) dL_dts = dL_dtn;
+    else if (q.tf_id == n_adj) dL_dts = dL_dtf;
+
+    const int64_t tbase = q.tbase;
+    const CellSink sink{row,
+                        G.spheres + 4 * (int64_t)cell,
+                        G.nsigmas + 4 * (int64_t)cell,
+                        G.sites + 3 * tbase,
+                        G.rgb + 3 * tbase,
+                        G.height + tbase,
+                        K};
+    const auto rgbs = colors.cell(cell, K);
+    f3 dL_dc, dL_dn;
+    float dL_dtn0, dL_dr = 0.0f;
+    texel_bwd<A, KT>(o, d, q.tn0, c, q.nrm, r, s.sites + 3 * tbase, rgbs, s.height + tbase, K, tx, dL_dcolor, dL_dts,
+                     sink, dL_dc, dL_dn, dL_dtn0, exact, dL_dr);
+    if (q.tn_id != n_adj) dL_dtn0 += dL_dtn;
+    for (int side = 0; side < 2; ++side) {
+        const int tid = side == 0 ? q.tn0_id : q.tf0_id;
+        const float dLdt = side == 0 ? dL_dtn0 : (q.tf_id != n_adj ? dL_dtf : 0.0f);
+        if (tid == -1) {
+            const SphereGrad sg = ray_sphere_bwd(o, d, c, r);
+            if (side == 0) {
+                dL_dc = dL_dc + sg.dn_dc * dLdt;
+                dL_dr += dLdt * sg.dn_dr;
+            } else {
+                dL_dc = dL_dc + sg.df_dc * dLdt;
+                dL_dr += dLdt * sg.df_dr;
+            }
+        } else if (tid >= 0 && tid < n_adj) {
+            const int adjc = s.adj[q.e0 + tid];
+            const float4 asp = s.spheres[adjc];
+            const FaceGrad fg = ray_pface_bwd(o, d, c, r, mk(asp.x, asp.y, asp.z), asp.w);
+            dL_dc = dL_dc + fg.dc * dLdt;
+            dL_dr += dLdt * fg.dr;
+            float *ga = G.spheres + 4 * (int64_t)adjc;
+            atomic_add(ga + 0, dLdt * fg.dcj.x);
+            atomic_add(ga + 1, dLdt * fg.dcj.y);
+            atomic_add(ga + 2, dLdt * fg.dcj.z);
+            atomic_add(ga + 3, dLdt * fg.drj);
+        }
+    }
+    sink.sphere(dL_dc, dL_dr);
+    sink.nsig(dL_dn, dL_dsigma);
+}
+
 // Backward by checkpoint and replay.  The forward's per-ray colour C and
 // log transmittance are the checkpoint.  The ray replays its walk in
 // forward order with the forward's arithmetic (so the cell sequence and
@@ -682,7 +769,8 @@ __device__ __forceinline__ int walk_ray_bwd(const SceneDev<A> &s, const RGB &col
             n_adj = e1 - e0;
             int exit_e = -1;
             float pt_far = 1e10f;
-            scan_row<F>(s.faces, s.E, e0, e1, [&](int e, f3 fn, float fw) {
+            // batches of 8 measured faster than 4 here (and slower in the forward)
+            scan_row<F, 8>(s.faces, s.E, e0, e1, [&](int e, f3 fn, float fw) {
                 float t, dp;
                 ray_pface_diff(o, d, fn, fw, t, dp);
                 if (dp >= 0.0f && t < pt_far) { exit_e = e; pt_far = t; }
@@ -722,60 +810,10 @@ __device__ __forceinline__ int walk_ray_bwd(const SceneDev<A> &s, const RGB &col
         // ---- gradients of the shaded segment
         const int slot = cache.assign(cell);
         if (cell == kNoCell) continue;
-        bytes += 2ull * (32ull + (uint64_t)K * 28ull);  // read-modify-write of the cell's gradients
-        const float dL_dLnext = dot(g, Cfin - rgb) + gL;
-        const float dL_dalpha = dot(g, tx.rgb) * trans;
-        const f3 dL_dcolor = (g * trans) * alpha;
-        const float dL_ddelta = dL_dLnext + (-dL_dalpha * e_dlt);
-        const float dL_dsigma = dL_ddelta * (-dt);
-        const float dL_dtn = dL_ddelta * sigma;
-        const float dL_dtf = dL_ddelta * (-sigma);
-
-        float dL_dts = 0.0f;
-        if (tn_id == n_adj) dL_dts = dL_dtn;
-        else if (tf_id == n_adj) dL_dts = dL_dtf;
-
-        const CellSink sink{cache.row(slot),
-                            G.spheres + 4 * (int64_t)cell,
-                            G.nsigmas + 4 * (int64_t)cell,
-                            G.sites + 3 * tbase,
-                            G.rgb + 3 * tbase,
-                            G.height + tbase,
-                            K};
-        const auto rgbs = colors.cell(cell, K);
-        f3 dL_dc, dL_dn;
-        float dL_dtn0, dL_dr = 0.0f;
-        texel_bwd<A, KT>(o, d, tn0, c, nrm, r, s.sites + 3 * tbase, rgbs, s.height + tbase, K, tx, dL_dcolor,
-                         dL_dts, sink, dL_dc, dL_dn, dL_dtn0, exact, dL_dr);
-        if (tn_id != n_adj) dL_dtn0 += dL_dtn;
-        for (int side = 0; side < 2; ++side) {
-            const int tid = side == 0 ? tn0_id : tf0_id;
-            const float dLdt = side == 0 ? dL_dtn0 : (tf_id != n_adj ? dL_dtf : 0.0f);
-            if (tid == -1) {
-                const SphereGrad sg = ray_sphere_bwd(o, d, c, r);
-                if (side == 0) {
-                    dL_dc = dL_dc + sg.dn_dc * dLdt;
-                    dL_dr += dLdt * sg.dn_dr;
-                } else {
-                    dL_dc = dL_dc + sg.df_dc * dLdt;
-                    dL_dr += dLdt * sg.df_dr;
-                }
-            } else if (tid >= 0 && tid < n_adj) {
-                const int adjc = s.adj[e0 + tid];
-                const float4 asp = s.spheres[adjc];
-                const FaceGrad fg = ray_pface_bwd(o, d, c, r, mk(asp.x, asp.y, asp.z), asp.w);
-                dL_dc = dL_dc + fg.dc * dLdt;
-                dL_dr += dLdt * fg.dr;
-                float *ga = G.spheres + 4 * (int64_t)adjc;
-                atomic_add(ga + 0, dLdt * fg.dcj.x);
-                atomic_add(ga + 1, dLdt * fg.dcj.y);
-                atomic_add(ga + 2, dLdt * fg.dcj.z);
-                atomic_add(ga + 3, dLdt * fg.drj);
-                bytes += 32;  // neighbour sphere gradient read-modify-write
-            }
-        }
-        sink.sphere(dL_dc, dL_dr);
-        sink.nsig(dL_dn, dL_dsigma);
+        bytes += segment_grad_bytes(K, tn0_id, tf0_id, n_adj);
+        segment_grads<A, F, KT>(s, colors, o, d, Seg{cell, c, nrm, r, sigma, tn0, dt, e_dlt, alpha, trans, rgb, tn_id,
+                                                     tf_id, tn0_id, tf0_id, e0, n_adj, tbase},
+                                tx, g, gL, Cfin, G, exact, cache.row(slot));
     }
     cache.flush_all();
     return status;
