@@ -40,9 +40,11 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work (0 = skip)")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--serve-requests", type=int, default=160, help="requests in the serving trace (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=240, help="requests in each serving trace (0 = skip)")
     ap.add_argument("--serve-load", type=float, default=0.6, help="offered load as a fraction of plain capacity")
     ap.add_argument("--serve-capacity", type=int, default=16384, help="8x8 tiles per serving step")
+    ap.add_argument("--serve-cells", type=int, default=3_000_000, help="cells of the single served scene")
+    ap.add_argument("--serve-scenes", type=int, default=8, help="scenes of the multi-scene serving leg (0 = skip)")
     ap.add_argument("--cell-order", choices=("morton", "input"), default="morton",
                     help="cells in Morton order (as the reference's training leaves them) or as generated")
     ap.add_argument("--adjacency", choices=("gpu", "qhull"), default="gpu",
@@ -178,13 +180,18 @@ def main():
         f"({fwd_bytes / fwd_ms / 1e6:.0f} GB/s alg), bwd {bwd_ms:.2f} ms ({bwd_bytes / bwd_ms / 1e6:.0f} GB/s alg); "
         f"visits/ray mean {visits.mean().item():.1f} max {int(visits.max().item())}; status {status}")
 
-    serving = None
-    if args.serve_requests > 0:
-        serving = serve_bench(scene, coeffs, pts, rad, sc, args)
-
     cpu = None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(sc, adj.numpy(), off.numpy(), faces, texel_rgb, cams_dev, f, gc, args.cpu_seconds)
+
+    serving = serving_multi = None
+    if args.serve_requests > 0:
+        # the serving legs build their own scenes: free the benchmark scene
+        del scene, grads, texel_rgb, faces, f, b, gc
+        torch.cuda.empty_cache()
+        serving = serve_bench(args)
+        if args.serve_scenes > 0:
+            serving_multi = serve_multi_bench(args)
 
     out = {
         "metric": "Mrays/s fwd+bwd (1M cells, 1080p) + p50/p99 render latency vs plain path",
@@ -228,40 +235,49 @@ def main():
         },
         "cpu_baseline": cpu,
         "serving": serving,
+        "serving_multi": serving_multi,
     }
     print(json.dumps(out), flush=True)
 
 
-def serve_bench(scene, coeffs, pts, rad, sc, args):
-    """Serving form on the resident 1M-cell scene: a Poisson stream of
-    forward requests, resolutions uniform over {256^2, 512^2, 1024^2,
-    1920x1080}, cameras on the radius-3 sphere.  The same trace is served by
-    the continuous-batching server and by the plain per-request path (SH
-    pre-pass + walk, one request at a time); the arrival rate is set to
-    serve_load x the plain path's measured capacity."""
+def build_served_scene(n, seed):
+    """A resident GMM scene of n cells (as the benchmark's: SH degree 3,
+    Morton-ordered cells, GPU power-diagram adjacency, packed walk faces)."""
+    import numpy as np
+    import torch
+
+    from powerfoam import raytrace as rt
+    from powerfoam import workloads
+    from powerfoam.geometry import morton_sort
+    from powerfoam.power import build_power_adjacency_gpu
+    from powerfoam.serving import ResidentScene
+
+    sc = workloads.gmm_scene(n, seed=seed)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    pts = t(sc["points"])
+    perm = morton_sort(pts).long()
+    pts, rad = pts[perm].contiguous(), t(sc["radii"])[perm].contiguous()
+    adj, off = build_power_adjacency_gpu(pts, rad)
+    faces = rt.walk_faces(pts, rad, adj, off)
+    return ResidentScene(pts, rad, t(sc["density"])[perm], t(sc["normals"])[perm], t(sc["texel_sites"])[perm],
+                         t(sc["texel_height"])[perm], adj, off, faces, sh=t(sc["sh"])[perm])
+
+
+def serve_trace(scenes, make, args, label):
+    """Serve one Poisson trace with the continuous-batching server and with
+    the plain per-request path (one request at a time, FIFO), at
+    serve_load x the plain path's measured capacity.  Returns the stats."""
     import copy
 
     import numpy as np
     import torch
 
-    from powerfoam import workloads
-    from powerfoam.serving import PlainRenderer, RenderRequest, RenderServer, ResidentScene, latency_stats
+    from powerfoam.serving import PlainRenderer, RenderServer, latency_stats
 
-    res = [(256, 256), (512, 512), (1024, 1024), (1920, 1080)]
-    rs = ResidentScene(pts, rad, scene.nsigmas[:, 3], scene.nsigmas[:, :3], scene.texel_sites, scene.texel_height,
-                       scene.adjacency, scene.offsets, scene.faces, sh=coeffs)
-    rng = np.random.default_rng(1000 + args.seed)
-
-    def make(n):
-        reqs = []
-        for k in range(n):
-            w, h = res[int(rng.integers(0, 4))]
-            cam = workloads.sphere_cameras(rng, 1, w, h)[0]
-            reqs.append(RenderRequest(scene="s", camera=cam, rid=k))
-        return reqs
-
-    plain = PlainRenderer({"s": rs})
-    warm = make(8)
+    rng = np.random.default_rng(2000 + args.seed)
+    plain = PlainRenderer(scenes)
+    # warm-up on a mix of every kind (the allocator's cache, graph capture)
+    warm = make(32)
     for r in warm:
         plain.render(r)
     torch.cuda.synchronize()
@@ -277,22 +293,20 @@ def serve_bench(scene, coeffs, pts, rad, sc, args):
     for r, a in zip(trace, arr):
         r.arrival = float(a)
     rays = sum(r.width * r.height for r in trace)
-    log(f"serving: {len(trace)} requests, rate {rate:.1f}/s (plain mean service {mean_service * 1e3:.2f} ms)")
-
+    log(f"{label}: {len(trace)} requests, rate {rate:.1f}/s (plain mean service {mean_service * 1e3:.2f} ms)")
     p_trace = copy.deepcopy(trace)
     p_wall = plain.run(p_trace)
     p_stats = latency_stats(p_trace)
-
     srv = RenderServer("cuda", memory_budget=64 << 30, tile_capacity=args.serve_capacity, quantum=args.serve_quantum)
-    srv.add_scene("s", rs)
-    w_trace = make(4)
-    srv.run(w_trace)  # warm-up (graph capture)
+    for name, scene in scenes.items():
+        srv.add_scene(name, scene)
+    srv.run(copy.deepcopy(warm))  # warm-up (graph capture, the allocator's cache)
     s_trace = copy.deepcopy(trace)
     srv.finished, srv.rejected = [], []
     s_wall = srv.run(s_trace)
     s_stats = latency_stats([r for r in s_trace if not r.rejected])
     # identical images: the served result equals the plain path's
-    same = all(torch.equal(a.color, b.color) for a, b in zip(s_trace[:8], p_trace[:8]))
+    same = all(torch.equal(a.color, b.color) for a, b in zip(s_trace[:8], p_trace[:8]) if not a.rejected)
     out = {
         "requests": len(trace), "rate_hz": round(rate, 2), "load": args.serve_load,
         "p50_ms": round(s_stats["p50_ms"], 3), "p99_ms": round(s_stats["p99_ms"], 3),
@@ -302,11 +316,75 @@ def serve_bench(scene, coeffs, pts, rad, sc, args):
         "rejected": len([r for r in s_trace if r.rejected]), "status": int(srv.status.item()),
         "tile_capacity": args.serve_capacity, "quantum": args.serve_quantum, "colour": srv.colour,
         "results_equal_plain": bool(same),
-        "workload": "1M-cell GMM scene resident, SH3 colour pre-pass per request, forward requests, "
-                    "resolutions uniform over 256^2/512^2/1024^2/1920x1080, Poisson arrivals",
     }
-    log(f"serving: p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms vs plain p50 {out['plain_p50_ms']} "
-        f"p99 {out['plain_p99_ms']} ms")
+    log(f"{label}: p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms vs plain p50 {out['plain_p50_ms']} "
+        f"p99 {out['plain_p99_ms']} ms, rejected {out['rejected']}")
+    return out
+
+
+RESOLUTIONS = [(256, 256), (512, 512), (1024, 1024), (1920, 1080)]
+
+
+def serve_bench(args):
+    """Serving form, one resident scene of --serve-cells cells (3M): a
+    Poisson stream of forward requests, resolutions uniform over {256^2,
+    512^2, 1024^2, 1920x1080}, cameras on the radius-3 sphere."""
+    import numpy as np
+
+    from powerfoam import workloads
+    from powerfoam.serving import RenderRequest
+
+    t0 = time.time()
+    scenes = {"s": build_served_scene(args.serve_cells, args.seed + 1)}
+    log(f"serving scene: {args.serve_cells} cells resident in {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(1000 + args.seed)
+
+    def make(n):
+        reqs = []
+        for k in range(n):
+            w, h = RESOLUTIONS[int(rng.integers(0, 4))]
+            reqs.append(RenderRequest(scene="s", camera=workloads.sphere_cameras(rng, 1, w, h)[0], rid=k))
+        return reqs
+
+    out = serve_trace(scenes, make, args, "serving")
+    out["workload"] = (f"{args.serve_cells}-cell GMM scene resident, SH3 colour pre-pass per request, forward "
+                       "requests, resolutions uniform over 256^2/512^2/1024^2/1920x1080, Poisson arrivals")
+    return out
+
+
+def serve_multi_bench(args):
+    """Serving form, several resident scenes under memory admission: cell
+    counts log-uniform in [1e5, 2e6], each request picks a scene uniformly,
+    resolutions as above, a quarter of the requests forward+backward against
+    a random target image, a 64 GB budget."""
+    import numpy as np
+    import torch
+
+    from powerfoam import workloads
+    from powerfoam.serving import RenderRequest
+
+    rng = np.random.default_rng(3000 + args.seed)
+    sizes = np.exp(rng.uniform(np.log(1e5), np.log(2e6), args.serve_scenes)).astype(int)
+    t0 = time.time()
+    scenes = {f"s{k}": build_served_scene(int(n), args.seed + 10 + k) for k, n in enumerate(sizes)}
+    log(f"serving scenes: {', '.join(str(int(n)) for n in sizes)} cells resident in {time.time() - t0:.1f}s")
+
+    def make(n):
+        reqs = []
+        for k in range(n):
+            w, h = RESOLUTIONS[int(rng.integers(0, 4))]
+            name = f"s{int(rng.integers(0, len(sizes)))}"
+            bwd = rng.uniform() < 0.25
+            tgt = torch.from_numpy(rng.uniform(0, 1, (h, w, 3)).astype(np.float32)) if bwd else None
+            reqs.append(RenderRequest(scene=name, camera=workloads.sphere_cameras(rng, 1, w, h)[0], rid=k,
+                                      mode="forward_backward" if bwd else "forward", target=tgt))
+        return reqs
+
+    out = serve_trace(scenes, make, args, "serving (multi-scene)")
+    out["scene_cells"] = [int(n) for n in sizes]
+    out["workload"] = (f"{len(sizes)} GMM scenes resident (cells log-uniform in [1e5, 2e6]), a scene picked "
+                       "uniformly per request, 25% forward+backward requests against random targets, "
+                       "resolutions uniform over 256^2/512^2/1024^2/1920x1080, Poisson arrivals, 64 GB budget")
     return out
 
 

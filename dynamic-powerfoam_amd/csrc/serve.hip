@@ -9,7 +9,8 @@
 // found by binary search over the table's step_tile0 prefix.  Forward
 // requests write colour and log transmittance; forward+backward requests
 // also replay the walk for the gradient of sum_pixels |C - target|^2
-// (checkpoint: the colour just computed, kept in registers).  The two
+// (checkpoint: the colour just computed, kept in registers), through the
+// wave's LDS gradient cache.  The two
 // kinds run in two launches of one step, so forward tiles are not held to
 // the occupancy of the backward's register footprint.
 #include "pf_device.h"
@@ -29,9 +30,8 @@ __device__ __forceinline__ int find_request(const pf_serve_request *reqs, int n,
     return lo;
 }
 
-// One ray of a request: forward, and with BWD the replayed backward of
-// sum |C - target|^2 (the checkpoint is the colour just computed).
-template <bool BWD, typename RGB>
+// One ray of a forward request.
+template <typename RGB>
 __device__ __forceinline__ int serve_ray(const SceneDev<float> &s, const RGB &colors, const pf_serve_request &rq,
                                          const Cam &cam, f3 d, int64_t pix, float thr, int max_steps) {
     const RayFwd r = walk_ray_fwd<float, PackedF16, 8>(s, colors, cam.eye, d, cam.start, thr, 0.5f, max_steps,
@@ -40,17 +40,43 @@ __device__ __forceinline__ int serve_ray(const SceneDev<float> &s, const RGB &co
     rq.color[3 * pix + 1] = r.rgb.y;
     rq.color[3 * pix + 2] = r.rgb.z;
     if (rq.log_t) rq.log_t[pix] = r.log_t;
-    int st = r.status;
-    if constexpr (BWD) {
+    return r.status;
+}
+
+// Gradient-cache slots of the forward+backward launch (as the walk
+// backward's: 9.9 KB of LDS per wave).
+constexpr int kServeSlots = 38;
+using ServeCache = CellCache<kServeSlots, 8>;
+
+// One ray of a forward+backward request: the forward, then the replayed
+// backward of sum |C - target|^2 (the checkpoint is the colour just
+// computed).  A wave serves one tile of one request, so its gradients go
+// through the wave's LDS cache (rows) into that request's arrays; every
+// lane takes part (valid == false: a pixel outside the image).
+template <typename RGB>
+__device__ __forceinline__ int serve_ray_bwd(const SceneDev<float> &s, const RGB &colors,
+                                             const pf_serve_request &rq, const Cam &cam, f3 d, int64_t pix,
+                                             float thr, int max_steps, float *rows, bool valid) {
+    RayFwd r{};
+    f3 gc = mk(0, 0, 0);
+    if (valid) {
+        r = walk_ray_fwd<float, PackedF16, 8>(s, colors, cam.eye, d, cam.start, thr, 0.5f, max_steps, nullptr, 0);
+        rq.color[3 * pix] = r.rgb.x;
+        rq.color[3 * pix + 1] = r.rgb.y;
+        rq.color[3 * pix + 2] = r.rgb.z;
+        if (rq.log_t) rq.log_t[pix] = r.log_t;
         const f3 tg = mk(rq.target[3 * pix], rq.target[3 * pix + 1], rq.target[3 * pix + 2]);
-        const f3 gc = 2.0f * (r.rgb - tg);
-        const GradDev G{rq.grads.spheres, rq.grads.nsigmas, rq.grads.texel_sites, rq.grads.texel_rgb,
-                        rq.grads.texel_height};
-        uint64_t bytes = 0;
-        st |= walk_ray_bwd<float, PackedF16, 8>(s, colors, cam.eye, d, cam.start, thr, max_steps, r.rgb, gc, 0.0f, G,
-                                               false, bytes);
+        gc = 2.0f * (r.rgb - tg);
     }
-    return st;
+    const GradDev G{rq.grads.spheres, rq.grads.nsigmas, rq.grads.texel_sites, rq.grads.texel_rgb,
+                    rq.grads.texel_height};
+    ServeCache cache;
+    cache.init(rows, G);
+    uint64_t bytes = 0;
+    const bool live = valid && (gc.x != 0.0f || gc.y != 0.0f || gc.z != 0.0f);
+    return r.status | walk_ray_bwd<float, PackedF16, 8, RGB, ServeCache>(s, colors, cam.eye, d, cam.start, thr,
+                                                                         max_steps, r.rgb, gc, 0.0f, G, false,
+                                                                         bytes, live, cache);
 }
 
 // Served scenes use f32 attributes, K = 8 and packed walk faces
@@ -72,17 +98,28 @@ __global__ __launch_bounds__(256) void serve_kernel(const pf_serve_scene *scenes
     const int tiles_h = (H + 7) / 8;
     const int pi = (tile % tiles_h) * 8 + (lane & 7);
     const int pj = (tile / tiles_h) * 8 + (lane >> 3);
-    if (pi >= H || pj >= W) return;
-    const int64_t pix = (int64_t)pi * W + pj;
+    const bool valid = pi < H && pj < W;
+    if (!BWD && !valid) return;
+    const int64_t pix = valid ? (int64_t)pi * W + pj : 0;
     const pf_serve_scene &sc = scenes[rq.scene];
     Cam cam = load_cam(reinterpret_cast<const float *>(&rq.cam));
     if (rq.start_cell) cam.start = __builtin_amdgcn_readfirstlane(*rq.start_cell);
     const f3 d = ray_dir(cam, pi, pj);
     const SceneDev<float> s = scene_dev<float>(sc.scene);
     const float *rgb = rq.texel_rgb ? rq.texel_rgb : reinterpret_cast<const float *>(sc.scene.texel_rgb);
+    const bool sh_in_walk = SH && !rq.texel_rgb && sc.sh;
     int st;
-    if (SH && !rq.texel_rgb && sc.sh) st = serve_ray<BWD>(s, SHRGB<DEG>{sc.sh, cam.eye}, rq, cam, d, pix, thr, max_steps);
-    else st = serve_ray<BWD>(s, MemRGB<float>{rgb}, rq, cam, d, pix, thr, max_steps);
+    if constexpr (BWD) {
+        __shared__ float rows[4][kServeSlots * ServeCache::kPitch];
+        float *wave_rows = rows[threadIdx.x >> 6];
+        if (sh_in_walk)
+            st = serve_ray_bwd(s, SHRGB<DEG>{sc.sh, cam.eye}, rq, cam, d, pix, thr, max_steps, wave_rows, valid);
+        else
+            st = serve_ray_bwd(s, MemRGB<float>{rgb}, rq, cam, d, pix, thr, max_steps, wave_rows, valid);
+    } else {
+        if (sh_in_walk) st = serve_ray(s, SHRGB<DEG>{sc.sh, cam.eye}, rq, cam, d, pix, thr, max_steps);
+        else st = serve_ray(s, MemRGB<float>{rgb}, rq, cam, d, pix, thr, max_steps);
+    }
     if (st && status) atomicOr(status, st);
 }
 

@@ -128,7 +128,7 @@ class ResidentScene:
 
 class RenderServer:
     def __init__(self, device="cuda", memory_budget=64 << 30, tile_capacity=16384, quantum=1024, use_graph=True,
-                 max_wait=2.0, transmittance_threshold=1e-3, max_steps=1 << 16, colour="prepass"):
+                 max_wait=2.0, transmittance_threshold=1e-3, max_steps=1 << 16, colour="prepass", bwd_weight=1):
         if colour not in ("prepass", "walk"):
             raise ValueError("colour must be 'prepass' or 'walk'")
         self.colour = colour
@@ -136,6 +136,7 @@ class RenderServer:
         self.budget = int(memory_budget)
         self.capacity = int(tile_capacity)
         self.quantum = int(quantum)
+        self.bwd_weight = max(int(bwd_weight), 1)
         self.use_graph = use_graph
         self.max_wait = max_wait
         self.thr = float(transmittance_threshold)
@@ -232,26 +233,34 @@ class RenderServer:
             r._next_tile = 0
             self.active.append(r)
 
+    def _cost(self, r):
+        return self.bwd_weight if r.mode == "forward_backward" else 1
+
     def _schedule(self):
         """Pick this step's tiles: up to quantum per active request in arrival
-        order, leftover capacity to the oldest requests."""
+        order, leftover capacity to the oldest requests.  Capacity and quantum
+        count forward tiles; a forward+backward tile counts bwd_weight of them
+        (it costs that much more), so a step's duration stays bounded whatever
+        the mix and a short request never waits behind a long step."""
         plan = []
         cap = self.capacity
         for r in self.active:
             if cap <= 0:
                 break
-            take = min(r.ntiles - r._next_tile, self.quantum, cap)
+            w = self._cost(r)
+            take = min(r.ntiles - r._next_tile, max(self.quantum // w, 1), cap // w)
             if take > 0:
                 plan.append([r, r._next_tile, take])
-                cap -= take
+                cap -= take * w
         for p in plan:
             if cap <= 0:
                 break
             r, b, take = p
-            more = min(r.ntiles - (b + take), cap)
+            w = self._cost(r)
+            more = min(r.ntiles - (b + take), cap // w)
             if more > 0:
                 p[2] += more
-                cap -= more
+                cap -= more * w
         return plan
 
     def _write_table(self, buf, plan):
