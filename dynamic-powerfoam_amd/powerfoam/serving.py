@@ -234,7 +234,7 @@ class RenderServer:
             self.active.append(r)
 
     def _cost(self, r):
-        return self.bwd_weight if r.mode == "forward_backward" else 1
+        return self.bwd_weight if getattr(r, "mode", "forward") == "forward_backward" else 1
 
     def _schedule(self):
         """Pick this step's tiles: up to quantum per active request in arrival
