@@ -40,10 +40,11 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work (0 = skip)")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--serve-requests", type=int, default=240, help="requests in each serving trace (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=480, help="requests in each serving trace (0 = skip)")
     ap.add_argument("--serve-load", type=float, default=0.6, help="offered load as a fraction of plain capacity")
     ap.add_argument("--serve-capacity", type=int, default=16384, help="8x8 tiles per serving step")
     ap.add_argument("--serve-cells", type=int, default=3_000_000, help="cells of the single served scene")
+    ap.add_argument("--raster", type=int, default=1, help="also time the tile rasterizer fwd+bwd (0 = skip)")
     ap.add_argument("--serve-scenes", type=int, default=8, help="scenes of the multi-scene serving leg (0 = skip)")
     ap.add_argument("--cell-order", choices=("morton", "input"), default="morton",
                     help="cells in Morton order (as the reference's training leaves them) or as generated")
@@ -184,6 +185,10 @@ def main():
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(sc, adj.numpy(), off.numpy(), faces, texel_rgb, cams_dev, f, gc, args.cpu_seconds)
 
+    raster = None
+    if args.raster:
+        raster = raster_bench(sc, pts, rad, texel_rgb[0], target, cam, args)
+
     serving = serving_multi = None
     if args.serve_requests > 0:
         # the serving legs build their own scenes: free the benchmark scene
@@ -234,10 +239,71 @@ def main():
             "traffic": profiled_traffic(dom) if (args.cells, W, H) == (1_000_000, 1920, 1080) else None,
         },
         "cpu_baseline": cpu,
+        "raster": raster,
         "serving": serving,
         "serving_multi": serving_multi,
     }
     print(json.dumps(out), flush=True)
+
+
+def raster_bench(sc, pts, rad, texel_rgb, target, cam, args):
+    """The pinhole tile rasterizer (Rasterizer.forward and its backward,
+    rasterize.py:892-1516) on the benchmark's scene, camera and loss, over
+    the scene's Cech complex (AABBTree.build_cech_complex, bvh.py:96-123, on
+    the GPU).  Timed with events on torch's stream, binning included."""
+    import types
+
+    import numpy as np
+    import torch
+
+    from powerfoam.bvh import AABBTree
+    from powerfoam.rasterize import Rasterizer
+
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    t0 = time.time()
+    tree = AABBTree("cuda")
+    tree.update(pts, rad)
+    adj, off = tree.build_cech_complex()
+    torch.cuda.synchronize()
+    cech_s = time.time() - t0
+    ra = types.SimpleNamespace(render_objective="volume", num_texel_sites=8, is_pinhole=True,
+                               disable_coop_prim_load=False, disable_coop_adj_load=False)
+    rz = Rasterizer(ra, "cuda")
+    params = [pts.clone().requires_grad_(True), rad.clone().requires_grad_(True),
+              t(sc["density"]).requires_grad_(True), t(sc["normals"]).requires_grad_(True),
+              t(sc["texel_sites"]).requires_grad_(True), texel_rgb.clone().requires_grad_(True),
+              t(sc["texel_height"]).requires_grad_(True)]
+    tgt = target[0]
+
+    def run(bwd):
+        out = rz.forward(cam, None, *params, adj, off, None, False)
+        if bwd:
+            ((out[0] - tgt) ** 2).sum().backward()
+        return out
+
+    for _ in range(2):
+        run(True)
+    torch.cuda.synchronize()
+    times = {}
+    for bwd in (False, True):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ms = []
+        for _ in range(3):
+            for p in params:
+                p.grad = None
+            ev[0].record()
+            run(bwd)
+            ev[1].record()
+            torch.cuda.synchronize()
+            ms.append(ev[0].elapsed_time(ev[1]))
+        times[bwd] = float(np.median(ms))
+    W, H = cam.width, cam.height
+    out = {"adjacency": "cech", "adjacency_edges": int(adj.numel()), "cech_build_s": round(cech_s, 2),
+           "fwd_ms": round(times[False], 3), "fwd_bwd_ms": round(times[True], 3),
+           "mrays_per_s": round(W * H / (times[True] * 1e-3) / 1e6, 3),
+           "workload": f"Rasterizer.forward + backward of an L2 loss, same scene, camera and target as the walk"}
+    log(f"raster: Cech {adj.numel()} edges in {cech_s:.2f}s; fwd {times[False]:.2f} ms, fwd+bwd {times[True]:.2f} ms")
+    return out
 
 
 def build_served_scene(n, seed):
