@@ -468,7 +468,14 @@ __global__ __launch_bounds__(64) void power_hull_kernel(const float *pts, const 
     const d3 pi = dd(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
     const double Li = (double)lifted[i];
     init_box(P, dd(-far, -far, -far), dd(far, far, far));
-    for (int q = 0; q < nh && P.nt > 0 && !P.overflow; ++q) {
+    // list neighbours first, alternating sides: the hull list is in cell
+    // order, and with cells in Morton order neighbours in the list are near
+    // in space, so the cell shrinks early and later planes miss its ball
+    for (int k = 1; k < nh && P.nt > 0 && !P.overflow; ++k) {
+        const int step = (k + 1) >> 1;
+        int q = (k & 1) ? h + step : h - step;
+        if (q >= nh) q -= nh;
+        if (q < 0) q += nh;
         const int j = hull[q];
         if (j == i) continue;
         const d3 dj = dd((double)pts[3 * j] - pi.x, (double)pts[3 * j + 1] - pi.y, (double)pts[3 * j + 2] - pi.z);
@@ -519,15 +526,44 @@ __global__ __launch_bounds__(256) void zero_bytes_kernel(unsigned char *p, int64
         p[i] = 0;
 }
 
-// Inclusive scan of n int32 into int64 by one block: each thread sums a
-// contiguous chunk, the block scans the chunk sums, each thread rescans.
-__global__ __launch_bounds__(1024) void pgrid_scan_kernel(const int *in, int64_t n, int64_t *out) {
+// Inclusive scan of n int32 into int64 in three launches: each block sums
+// a chunk of kScanChunk values, one block scans the chunk sums (exclusive,
+// in place), and each block rescans its chunk from that offset.
+constexpr int kScanChunk = 4096;  // values per block: 256 threads x 16
+
+__host__ __device__ inline int64_t scan_chunks(int64_t G) { return (G + 1 + kScanChunk - 1) / kScanChunk; }
+
+__device__ __forceinline__ int64_t block_sum_256(int64_t v, int64_t *part) {
+    const int t = threadIdx.x;
+    part[t] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off) part[t] += part[t + off];
+        __syncthreads();
+    }
+    const int64_t total = part[0];
+    __syncthreads();
+    return total;
+}
+
+__global__ __launch_bounds__(256) void scan_chunk_sums_kernel(const int *in, int64_t n, int64_t *sums) {
+    __shared__ int64_t part[256];
+    const int64_t b = (int64_t)blockIdx.x * kScanChunk;
+    int64_t v = 0;
+    for (int k = threadIdx.x; k < kScanChunk; k += 256)
+        if (b + k < n) v += in[b + k];
+    const int64_t total = block_sum_256(v, part);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of m chunk sums in place, by one block of 1024 threads.
+__global__ __launch_bounds__(1024) void scan_sums_kernel(int64_t *sums, int64_t m) {
     __shared__ int64_t part[1024];
     const int t = threadIdx.x;
-    const int64_t per = (n + 1023) / 1024;
-    const int64_t b = t * per, e = b + per < n ? b + per : n;
+    const int64_t per = (m + 1023) / 1024;
+    const int64_t b = t * per, e = b + per < m ? b + per : m;
     int64_t s = 0;
-    for (int64_t i = b; i < e; ++i) s += in[i];
+    for (int64_t i = b; i < e; ++i) s += sums[i];
     part[t] = s;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
@@ -538,9 +574,36 @@ __global__ __launch_bounds__(1024) void pgrid_scan_kernel(const int *in, int64_t
     }
     int64_t run = t > 0 ? part[t - 1] : 0;
     for (int64_t i = b; i < e; ++i) {
-        run += in[i];
-        out[i] = run;
+        const int64_t x = sums[i];
+        sums[i] = run;
+        run += x;
     }
+}
+
+// Each thread scans 16 consecutive values of the block's chunk; the block
+// scans the 256 thread totals in LDS.
+__global__ __launch_bounds__(256) void scan_chunks_kernel(const int *in, int64_t n, const int64_t *sums,
+                                                          int64_t *out) {
+    __shared__ int64_t part[256];
+    const int t = threadIdx.x;
+    const int64_t b = (int64_t)blockIdx.x * kScanChunk + (int64_t)t * 16;
+    int64_t v = 0;
+    for (int k = 0; k < 16; ++k)
+        if (b + k < n) v += in[b + k];
+    part[t] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int64_t x = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    int64_t run = sums[blockIdx.x] + (t > 0 ? part[t - 1] : 0);
+    for (int k = 0; k < 16; ++k)
+        if (b + k < n) {
+            run += in[b + k];
+            out[b + k] = run;
+        }
 }
 
 __global__ __launch_bounds__(256) void pgrid_scatter_kernel(int64_t N, const int *key, const int64_t *gstart,
@@ -572,24 +635,35 @@ inline bool make_grid(const pf_cech_grid *in, PowerGrid &g, int64_t &occ_bytes) 
 
 using namespace pf;
 
-extern "C" int64_t pf_power_occupancy_bytes(const pf_cech_grid *grid) {
+// Workspace layout: the occupancy pyramid, then (8-byte aligned) the bucket
+// scan's chunk sums.
+static int64_t workspace_layout(const PowerGrid &g, int64_t occ_bytes, int64_t &sums_offset) {
+    sums_offset = (occ_bytes + 7) & ~(int64_t)7;
+    return sums_offset + 8 * scan_chunks(grid_size(g, 0));
+}
+
+extern "C" int64_t pf_power_workspace_bytes(const pf_cech_grid *grid) {
     PowerGrid g;
-    int64_t bytes = 0;
-    if (!grid || !make_grid(grid, g, bytes)) return -1;
-    return bytes > 0 ? bytes : 1;
+    int64_t occ = 0, sums_offset = 0;
+    if (!grid || !make_grid(grid, g, occ)) return -1;
+    return workspace_layout(g, occ, sums_offset);
 }
 
 extern "C" int pf_power_cells(const float *points, const float *lifted, int64_t num_cells, const pf_cech_grid *grid,
                               int32_t *grid_counts, int32_t *grid_fill, int32_t *cell_key, int64_t *grid_start,
-                              int32_t *grid_items, uint8_t *occupancy, int64_t occupancy_bytes, const float *box,
+                              int32_t *grid_items, void *workspace, int64_t workspace_bytes, const float *box,
                               float r_max, float lift_slack, int32_t max_nb, int32_t *lists, int32_t *counts,
                               int32_t *flags, void *stream) {
     if (!points || !lifted || num_cells <= 0 || num_cells >= 0x7fffffff || !grid || !grid_counts || !grid_fill ||
-        !cell_key || !grid_start || !grid_items || !occupancy || !box || !lists || !counts || !flags)
+        !cell_key || !grid_start || !grid_items || !workspace || !box || !lists || !counts || !flags)
         return PF_ERR_ARG;
     PowerGrid g;
-    int64_t need = 0;
-    if (max_nb <= 0 || max_nb > kMaxPlanes || !make_grid(grid, g, need) || occupancy_bytes < need) return PF_ERR_ARG;
+    int64_t need = 0, sums_offset = 0;
+    if (max_nb <= 0 || max_nb > kMaxPlanes || !make_grid(grid, g, need) ||
+        workspace_bytes < workspace_layout(g, need, sums_offset))
+        return PF_ERR_ARG;
+    unsigned char *occupancy = static_cast<unsigned char *>(workspace);
+    int64_t *scan_sums = reinterpret_cast<int64_t *>(occupancy + sums_offset);
     const int64_t G = grid_size(g, 0);
     hipStream_t st = (hipStream_t)stream;
     const unsigned nb = (unsigned)((num_cells + 255) / 256);
@@ -600,7 +674,12 @@ extern "C" int pf_power_cells(const float *points, const float *lifted, int64_t 
     }
     hipLaunchKernelGGL(pgrid_count_kernel, dim3(nb), dim3(256), 0, st, points, num_cells, g, grid_counts, cell_key);
     hipLaunchKernelGGL(pgrid_mark_kernel, dim3(nb), dim3(256), 0, st, points, num_cells, g, occupancy);
-    hipLaunchKernelGGL(pgrid_scan_kernel, dim3(1), dim3(1024), 0, st, grid_counts, G + 1, grid_start);
+    const int64_t chunks = scan_chunks(G);
+    hipLaunchKernelGGL(scan_chunk_sums_kernel, dim3((unsigned)chunks), dim3(256), 0, st, grid_counts, G + 1,
+                       scan_sums);
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, st, scan_sums, chunks);
+    hipLaunchKernelGGL(scan_chunks_kernel, dim3((unsigned)chunks), dim3(256), 0, st, grid_counts, G + 1, scan_sums,
+                       grid_start);
     hipLaunchKernelGGL(pgrid_scatter_kernel, dim3(nb), dim3(256), 0, st, num_cells, cell_key, grid_start, grid_fill,
                        grid_items);
     hipLaunchKernelGGL(power_cells_kernel, dim3((unsigned)((num_cells + 63) / 64)), dim3(64), 0, st, points, lifted,

@@ -176,7 +176,7 @@ _SIGS = {
         [c_void_p, c_void_p, c_int64, ctypes.POINTER(PfCechGrid), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
          c_void_p],
     ),
-    "pf_power_occupancy_bytes": (c_int64, [ctypes.POINTER(PfCechGrid)]),
+    "pf_power_workspace_bytes": (c_int64, [ctypes.POINTER(PfCechGrid)]),
     "pf_morton_codes": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "pf_power_cells": (
         c_int32,

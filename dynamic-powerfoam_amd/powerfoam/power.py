@@ -96,16 +96,16 @@ def build_power_adjacency_gpu(points, radii, margin=None, sites_per_cell=3.0, fa
     key = torch.empty(N, dtype=torch.int32, device=dev)
     gstart = torch.empty(G + 1, dtype=torch.int64, device=dev)
     items = torch.empty(N, dtype=torch.int32, device=dev)
-    occ_bytes = int(lib.pf_power_occupancy_bytes(ctypes.byref(g)))
-    if occ_bytes < 0:
+    ws_bytes = int(lib.pf_power_workspace_bytes(ctypes.byref(g)))
+    if ws_bytes < 0:
         raise ValueError(f"bucket grid {dims} out of range")
-    occ = torch.empty(occ_bytes, dtype=torch.uint8, device=dev)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     lists = torch.empty(N, MAX_NEIGHBOURS, dtype=torch.int32, device=dev)
     counts = torch.empty(N, dtype=torch.int32, device=dev)
     flags = torch.empty(N, dtype=torch.int32, device=dev)
     P = _lib.ptr
     rc = lib.pf_power_cells(P(pts), P(L), N, ctypes.byref(g), P(gcount), P(gfill), P(key), P(gstart), P(items),
-                            P(occ), occ_bytes, box.data_ptr(), rmax, slack, MAX_NEIGHBOURS, P(lists), P(counts), P(flags),
+                            P(ws), ws_bytes, box.data_ptr(), rmax, slack, MAX_NEIGHBOURS, P(lists), P(counts), P(flags),
                             _lib.stream_ptr())
     _lib.check(rc, "pf_power_cells")
     if bool(((flags & 2) != 0).any().item()):

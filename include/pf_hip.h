@@ -404,7 +404,8 @@ int pf_serve_step(const pf_serve_scene *scenes,
  * buckets.  grid (cell size and dims over the sites' bounds, dims <= 2^20
  * per axis) is chosen by the caller; grid_counts (G+1) and grid_fill (G)
  * zeroed by the caller, cell_key and grid_items N, grid_start G+1;
- * occupancy: pf_power_occupancy_bytes(grid) bytes (any contents).
+ * workspace: pf_power_workspace_bytes(grid) bytes (any contents; the
+ * occupancy pyramid and the bucket scan's partial sums).
  * lists (N, max_nb) int32 (rows sorted ascending), counts (N), flags (N):
  * bit 0 the cell meets the box boundary (its faces outside the box are
  * those of the diagram of the flagged sites alone), bit 1 capacity
@@ -413,12 +414,12 @@ int pf_serve_step(const pf_serve_scene *scenes,
  * computed it (the reference lifts in f32, benchmark.py:29-32); the
  * bisector of i and j is 2 x.(p_j - p_i) <= L_j - L_i.  r_max: largest
  * radius; lift_slack >= |L - (|p|^2 - r^2)| (the lift's rounding).       */
-int64_t pf_power_occupancy_bytes(const pf_cech_grid *grid);
+int64_t pf_power_workspace_bytes(const pf_cech_grid *grid);
 int pf_power_cells(const float *points, const float *lifted, int64_t num_cells,
                    const pf_cech_grid *grid, int32_t *grid_counts,
                    int32_t *grid_fill, int32_t *cell_key, int64_t *grid_start,
-                   int32_t *grid_items, uint8_t *occupancy,
-                   int64_t occupancy_bytes, const float *box, float r_max,
+                   int32_t *grid_items, void *workspace,
+                   int64_t workspace_bytes, const float *box, float r_max,
                    float lift_slack, int32_t max_nb, int32_t *lists,
                    int32_t *counts, int32_t *flags, void *stream);
 
