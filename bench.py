@@ -360,12 +360,18 @@ def serve_trace(scenes, make, args, label):
         r.arrival = float(a)
     rays = sum(r.width * r.height for r in trace)
     log(f"{label}: {len(trace)} requests, rate {rate:.1f}/s (plain mean service {mean_service * 1e3:.2f} ms)")
+    def consume(r):  # a client takes the gradients and lets them go
+        r.grads = None
+
+    plain.on_finish = consume
     p_trace = copy.deepcopy(trace)
     p_wall = plain.run(p_trace)
     p_stats = latency_stats(p_trace)
-    srv = RenderServer("cuda", memory_budget=64 << 30, tile_capacity=args.serve_capacity, quantum=args.serve_quantum)
+    srv = RenderServer("cuda", memory_budget=64 << 30, tile_capacity=args.serve_capacity, quantum=args.serve_quantum,
+                       oldest_share=getattr(args, "serve_oldest_share", 0.0))
     for name, scene in scenes.items():
         srv.add_scene(name, scene)
+    srv.on_finish = consume
     srv.run(copy.deepcopy(warm))  # warm-up (graph capture, the allocator's cache)
     s_trace = copy.deepcopy(trace)
     srv.finished, srv.rejected = [], []

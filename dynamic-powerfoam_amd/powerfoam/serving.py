@@ -128,7 +128,8 @@ class ResidentScene:
 
 class RenderServer:
     def __init__(self, device="cuda", memory_budget=64 << 30, tile_capacity=16384, quantum=1024, use_graph=True,
-                 max_wait=2.0, transmittance_threshold=1e-3, max_steps=1 << 16, colour="prepass", bwd_weight=1):
+                 max_wait=2.0, transmittance_threshold=1e-3, max_steps=1 << 16, colour="prepass", bwd_weight=1,
+                 oldest_share=0.0):
         if colour not in ("prepass", "walk"):
             raise ValueError("colour must be 'prepass' or 'walk'")
         self.colour = colour
@@ -137,6 +138,8 @@ class RenderServer:
         self.capacity = int(tile_capacity)
         self.quantum = int(quantum)
         self.bwd_weight = max(int(bwd_weight), 1)
+        self.oldest_share = float(oldest_share)
+        self.on_finish = None  # called with each finished request (e.g. to consume and drop its gradients)
         self.use_graph = use_graph
         self.max_wait = max_wait
         self.thr = float(transmittance_threshold)
@@ -244,11 +247,14 @@ class RenderServer:
         the mix and a short request never waits behind a long step."""
         plan = []
         cap = self.capacity
-        for r in self.active:
+        for k, r in enumerate(self.active):
             if cap <= 0:
                 break
             w = self._cost(r)
-            take = min(r.ntiles - r._next_tile, max(self.quantum // w, 1), cap // w)
+            quantum = self.quantum
+            if k == 0:  # the oldest request may take a larger share
+                quantum = max(quantum, int(self.oldest_share * self.capacity))
+            take = min(r.ntiles - r._next_tile, max(quantum // w, 1), cap // w)
             if take > 0:
                 plan.append([r, r._next_tile, take])
                 cap -= take * w
@@ -375,6 +381,8 @@ class RenderServer:
                 r._texel_rgb = None  # its last step has completed
                 self.reserved -= r._bytes
                 self.finished.append(r)
+                if self.on_finish is not None:
+                    self.on_finish(r)
 
     def finish_gradients(self, r):
         """dL/dSH of a finished forward+backward request (SH backward of its
@@ -418,6 +426,7 @@ class PlainRenderer:
     def __init__(self, scenes, transmittance_threshold=1e-3):
         self.scenes = scenes
         self.thr = transmittance_threshold
+        self.on_finish = None  # as RenderServer.on_finish
 
     def render(self, r: RenderRequest):
         sc = self.scenes[r.scene]
@@ -455,6 +464,8 @@ class PlainRenderer:
             self.render(r)
             torch.cuda.synchronize()
             r.done = time.perf_counter()
+            if self.on_finish is not None:
+                self.on_finish(r)
         return time.perf_counter() - t0
 
 
