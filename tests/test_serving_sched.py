@@ -15,9 +15,10 @@ def _fake(ntiles_list):
     return reqs
 
 
-def _plan(active, capacity, quantum):
+def _plan(active, capacity, quantum, oldest_share=0.0, bwd_weight=1):
     srv = RenderServer.__new__(RenderServer)
     srv.active, srv.capacity, srv.quantum = active, capacity, quantum
+    srv.oldest_share, srv.bwd_weight = oldest_share, bwd_weight
     return srv._schedule()
 
 
@@ -47,3 +48,19 @@ def test_every_tile_exactly_once():
         active = [r for r in active if r._next_tile < r.ntiles]
     for r in reqs:
         assert seen[r.rid] == list(range(r.ntiles))
+
+
+def test_oldest_share_and_backward_weight():
+    # the oldest request may take up to oldest_share of the capacity first
+    a = _fake([32400, 1024, 4096])
+    takes = {p[0].rid: p[2] for p in _plan(a, 4096, 1024, oldest_share=0.5)}
+    assert takes == {0: 2048, 1: 1024, 2: 1024}
+    takes = {p[0].rid: p[2] for p in _plan(_fake([32400, 1024, 4096]), 4096, 1024, oldest_share=1.0)}
+    assert takes == {0: 4096}
+    # a forward+backward tile counts bwd_weight forward tiles of capacity and quantum
+    b = _fake([32400, 1024])
+    b[0].mode = "forward_backward"
+    takes = {p[0].rid: p[2] for p in _plan(b, 4096, 1024, bwd_weight=4)}
+    # 256 (quantum / 4) for the backward request, 1024 for the forward one,
+    # then the remaining 2048 of capacity buys the oldest 512 more tiles
+    assert takes == {0: 768, 1: 1024}
