@@ -9,10 +9,13 @@ over up to ``tile_capacity`` 8x8 tiles drawn from all active requests
 oldest), so requests that arrive while others are on the GPU join at the
 next step and their tiles run in the same launch.  The request table lives
 in pinned host memory; each step copies it to the device and launches the
-kernels, and both are captured once in a HIP graph per table buffer (two
-buffers, so the host fills one while the GPU runs the other; each buffer
-has its own stream, so one step's last tiles overlap the next step's
-first).
+kernels, and both are captured once in a HIP graph per table buffer.  The
+buffers rotate (``streams``, default 4), so the host fills one while the
+GPU runs the others, and each has its own default-priority stream, so one
+step's last tiles overlap the next step's first.  With two streams, runs
+came in two modes: in some the two streams shared a hardware queue, the
+steps stopped overlapping and the p99 of a mixed trace doubled; with four,
+consecutive steps land on different queues.
 
 Colour: with ``colour="prepass"`` (default) each admitted request of an
 SH-coloured scene gets its own texel colours from one SH pre-pass for its
@@ -129,7 +132,7 @@ class ResidentScene:
 class RenderServer:
     def __init__(self, device="cuda", memory_budget=64 << 30, tile_capacity=16384, quantum=1024, use_graph=True,
                  max_wait=2.0, transmittance_threshold=1e-3, max_steps=1 << 16, colour="prepass", bwd_weight=1,
-                 oldest_share=0.0):
+                 oldest_share=0.0, streams=4):
         if colour not in ("prepass", "walk"):
             raise ValueError("colour must be 'prepass' or 'walk'")
         self.colour = colour
@@ -152,7 +155,7 @@ class RenderServer:
         self.inflight = deque()  # (event, [requests completing in this step])
         self.finished = []
         self.rejected = []
-        self.nbuf = 2
+        self.nbuf = int(streams)
         self.max_reqs = 256
         self._tables_dirty = True
         self._graphs = None
