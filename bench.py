@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--adjacency", choices=("gpu", "qhull"), default="gpu",
                     help="power-diagram adjacency from the GPU builder or the CPU Qhull lift (same edges)")
     ap.add_argument("--serve-quantum", type=int, default=1024, help="tiles per request per step before leftovers")
+    ap.add_argument("--record-cap", type=int, default=64,
+                    help="contributing segments per ray the forward records for the backward (0 = replay)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -124,7 +126,7 @@ def main():
         shading.sh_forward(coeffs, 3, sites, cams_dev, out=texel_rgb)
         if record:
             ev[0].record()
-        f = rt.launch_walk_forward(scene, cams_dev, W, H, want_depth=False)
+        f = rt.launch_walk_forward(scene, cams_dev, W, H, want_depth=False, record_cap=args.record_cap)
         if record:
             ev[1].record()
         gc = 2.0 * (f["color"] - target)
@@ -133,7 +135,8 @@ def main():
         g_coeffs.zero_()
         if record:
             ev[2].record()
-        b = rt.launch_walk_backward(scene, cams_dev, W, H, f["color"], f["log_t"], gc, None, grads=grads)
+        b = rt.launch_walk_backward(scene, cams_dev, W, H, f["color"], f["log_t"], gc, None, grads=grads,
+                                    record=f["record"])
         if record:
             ev[3].record()
         shading.sh_backward(coeffs, 3, sites, cams_dev, grads["texel_rgb"][None], grad_coeffs=g_coeffs)
@@ -161,13 +164,21 @@ def main():
 
     # algorithmic bytes from the kernels' own per-ray counters (untimed pass)
     shading.sh_forward(coeffs, 3, sites, cams_dev, out=texel_rgb)
-    f = rt.launch_walk_forward(scene, cams_dev, W, H, want_depth=False, want_sequences=True, want_bytes=True)
+    f = rt.launch_walk_forward(scene, cams_dev, W, H, want_depth=False, want_sequences=True, want_bytes=True,
+                               record_cap=args.record_cap)
     gc = 2.0 * (f["color"] - target)
     for k in GRAD_KEYS:
         grads[k].zero_()
     b = rt.launch_walk_backward(scene, cams_dev, W, H, f["color"], f["log_t"], gc, None, grads=grads,
-                                want_bytes=True)
+                                want_bytes=True, record=f["record"])
     torch.cuda.synchronize()
+    segs = None
+    if f["record"] is not None:
+        cnt = rt.record_counts(f["record"], 1, W, H).double()
+        segs = dict(cap=args.record_cap, mean=round(cnt.mean().item(), 2),
+                    p99=float(torch.quantile(cnt.flatten()[::7], 0.99).item()), max=int(cnt.max().item()),
+                    replayed_rays=int((cnt > args.record_cap).sum().item()))
+        log(f"record: contributing segments per ray {segs}")
     fwd_bytes = float(f["alg_bytes"].sum().item())
     bwd_bytes = float(b["alg_bytes"].sum().item())
     visits = f["visits"].double()
@@ -227,6 +238,7 @@ def main():
             "walk_bwd_ms": round(bwd_ms, 4),
             "walk_fwd_alg_bytes": fwd_bytes,
             "walk_bwd_alg_bytes": bwd_bytes,
+            "record_segments": segs,
             "status": status,
         },
         "roofline": {

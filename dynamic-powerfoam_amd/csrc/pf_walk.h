@@ -386,16 +386,36 @@ struct RayFwd {
     float log_t, depth;
     int visits, status;
     uint64_t hash, bytes;
+    int segs;  // contributing segments (recorded or not)
 };
 
-// raytrace.py:157-241; depth as rasterize.py:871-877.
-template <typename A, typename F, int KT, typename RGB>
+// A ray's record of contributing segments: segment k goes to rec[k * stride]
+// as {cell, face-bounded t_near bits, face-bounded t_far bits,
+// near face id (low 16 bits) | far face id (high 16 bits)}; ids index the
+// cell's CSR row, -1 for the sphere, -2 for a sphere bound behind the
+// origin.  Segments past cap are counted but not stored.
+struct SegRecord {
+    int4 *rec;
+    int64_t stride;
+    int cap;
+};
+constexpr uint64_t kSegRecordBytes = 16;
+
+__device__ __forceinline__ int4 pack_segment(int cell, float tn, float tf, int tn_id, int tf_id) {
+    return make_int4(cell, __float_as_int(tn), __float_as_int(tf),
+                     (int)(((uint32_t)tn_id & 0xffffu) | ((uint32_t)tf_id << 16)));
+}
+
+// raytrace.py:157-241; depth as rasterize.py:871-877.  REC: also write
+// the record of contributing segments the recorded backward reads.
+template <typename A, typename F, int KT, typename RGB, bool REC = false>
 __device__ __forceinline__ RayFwd walk_ray_fwd(const SceneDev<A> &s, const RGB &colors, f3 o, f3 d, int start,
-                                               float thr, float depth_q, int max_steps, int *seq, int seq_cap) {
+                                               float thr, float depth_q, int max_steps, int *seq, int seq_cap,
+                                               SegRecord rec = SegRecord{nullptr, 0, 0}) {
     constexpr uint64_t kVisitBytes = WalkBytes<A, F>::visit, kFaceBytes = WalkBytes<A, F>::face,
                        kTexelBytes = WalkBytes<A, F>::texel;
     const int K = KT > 0 ? KT : s.K;
-    RayFwd out{mk(0, 0, 0), 0.0f, 0.0f, 0, 0, kFnvOffset, 0};
+    RayFwd out{mk(0, 0, 0), 0.0f, 0.0f, 0, 0, kFnvOffset, 0, 0};
     bool depth_found = false;
     int prim = start;
     CycleGuard guard;
@@ -417,6 +437,12 @@ __device__ __forceinline__ RayFwd walk_ray_fwd(const SceneDev<A> &s, const RGB &
         SphereHit sh = ray_sphere(o, d, c, r);
         if (length(c - o) < 4.0f * r) sh.hit = false;
         float tn = sh.tn, tf = sh.tf;
+        // the bounding faces, tracked as walk_ray_bwd's replay tracks them
+        int tn_id = 0, tf_id = 0;
+        if constexpr (REC) {
+            tn_id = tn < 1e-8f ? -2 : -1;
+            tf_id = tf < 1e-8f ? -2 : -1;
+        }
 
         const int e0 = s.off[prim];
         const int e1 = s.off[prim + 1];
@@ -427,6 +453,10 @@ __device__ __forceinline__ RayFwd walk_ray_fwd(const SceneDev<A> &s, const RGB &
             float t, dp;
             ray_pface_diff(o, d, fn, fw, t, dp);
             if (dp >= 0.0f && t < pt_far) { exit_e = e; pt_far = t; }
+            if constexpr (REC) {
+                if (dp >= 0.0f && t < tf) tf_id = e - e0;
+                if (dp < 0.0f && t > tn) tn_id = e - e0;
+            }
             tf = dp >= 0.0f ? fmin_(t, tf) : tf;
             tn = dp < 0.0f ? fmax_(t, tn) : tn;
         });
@@ -437,11 +467,20 @@ __device__ __forceinline__ RayFwd walk_ray_fwd(const SceneDev<A> &s, const RGB &
         out.bytes += (uint64_t)K * kTexelBytes;
         const TexFwdT<KT> tx = texel_fwd<A, KT>(o, d, tn, c, nrm, r, s.sites + 3 * tbase, colors.cell(prim, K),
                                           s.height + tbase, K);
+        const float tn_face = tn, tf_face = tf;
         tf = tx.dp >= 0.0f ? fmin_(tx.t_surf, tf) : tf;
         tn = tx.dp < 0.0f ? fmax_(tx.t_surf, tn) : tn;
+        const int cell = prim;
         prim = next;
         const float dt = tf - tn;
         if (dt > 0.0f) {
+            if constexpr (REC) {
+                if (out.segs < rec.cap) {
+                    rec.rec[out.segs * rec.stride] = pack_segment(cell, tn_face, tf_face, tn_id, tf_id);
+                    out.bytes += kSegRecordBytes;
+                }
+                ++out.segs;
+            }
             const float dlt = (-sigma) * dt;
             const float alpha = 1.0f - pf_expf(dlt);
             out.rgb.x = __builtin_fmaf(tx.rgb.x * alpha, trans, out.rgb.x);
@@ -475,8 +514,11 @@ struct CellSink {
     float *gs, *gn, *gsite, *grgb, *gh;
     int K;
     __device__ __forceinline__ void add(int off, float *g, float v) const {
-        if (lds) lds_add(lds + off, v);  // adding 0 to the row is harmless; flushes skip zeros
-        else if (v != 0.0f) atomic_add(g, v);
+        // zero contributions are skipped: an LDS float atomic costs per
+        // active lane, and many of a segment's terms are exactly zero
+        if (v == 0.0f) return;
+        if (lds) lds_add(lds + off, v);
+        else atomic_add(g, v);
     }
     __device__ __forceinline__ void sphere(f3 dc, float dr) const {
         add(0, gs + 0, dc.x);
@@ -817,6 +859,77 @@ __device__ __forceinline__ int walk_ray_bwd(const SceneDev<A> &s, const RGB &col
     }
     cache.flush_all();
     return status;
+}
+
+// Backward from the forward's record of contributing segments (SegRecord):
+// the same per-segment arithmetic as walk_ray_bwd's replay, from the same
+// face-bounded t values, so every segment, the running colour and the
+// transmittance are bit-identical; no face row is read.  count is the
+// ray's number of contributing segments, all of them in the record (a ray
+// whose count exceeds the record's cap goes through walk_ray_bwd instead).
+template <typename A, typename F, int KT, typename RGB, typename Cache = LaneCache>
+__device__ __forceinline__ void walk_rec_bwd(const SceneDev<A> &s, const RGB &colors, f3 o, f3 d, const int4 *rec,
+                                             int64_t stride, int count, f3 Cfin, f3 g, float gL, const GradDev &G,
+                                             bool exact, uint64_t &bytes, bool live, Cache cache = Cache{}) {
+    constexpr uint64_t kVisitBytes = WalkBytes<A, F>::visit, kTexelBytes = WalkBytes<A, F>::texel;
+    const int K = KT > 0 ? KT : s.K;
+    f3 rgb = mk(0, 0, 0);
+    float log_t = 0.0f;
+    for (int k = 0;; ++k) {
+        const bool act = live && k < count;
+        if constexpr (Cache::kWave) {
+            if (!__any(act)) break;
+        } else {
+            if (!act) break;
+        }
+        int cell = kNoCell;
+        f3 c = mk(0, 0, 0), nrm = mk(0, 0, 0);
+        float r = 1.0f, sigma = 0.0f, tn0 = 0.0f, dt = 0.0f, e_dlt = 1.0f, alpha = 0.0f, trans = 0.0f;
+        int tn_id = -1, tf_id = -1, tn0_id = -1, tf0_id = -1, e0 = 0, n_adj = 0;
+        int64_t tbase = 0;
+        TexFwdT<KT> tx{};
+        if (act) {
+            const int4 q = rec[k * stride];
+            cell = q.x;
+            float tn = __int_as_float(q.y), tf = __int_as_float(q.z);
+            tn_id = (int)(int16_t)(q.w & 0xffff);
+            tf_id = q.w >> 16;
+            trans = pf_expf(log_t);
+            const float4 sp = s.spheres[cell];
+            ld4(s.nsig + 4 * (int64_t)cell, nrm, sigma);
+            c = mk(sp.x, sp.y, sp.z);
+            r = sp.w;
+            e0 = s.off[cell];
+            n_adj = s.off[cell + 1] - e0;
+            tbase = (int64_t)cell * K;
+            bytes += kSegRecordBytes + kVisitBytes + (uint64_t)K * kTexelBytes;
+            tn0 = tn;
+            tn0_id = tn_id;
+            tf0_id = tf_id;
+            tx = texel_fwd<A, KT>(o, d, tn, c, nrm, r, s.sites + 3 * tbase, colors.cell(cell, K), s.height + tbase,
+                                  K);
+            if (tx.dp >= 0.0f && tx.t_surf < tf) tf_id = n_adj;
+            tf = tx.dp >= 0.0f ? fmin_(tx.t_surf, tf) : tf;
+            if (tx.dp < 0.0f && tx.t_surf > tn) tn_id = n_adj;
+            tn = tx.dp < 0.0f ? fmax_(tx.t_surf, tn) : tn;
+            dt = tf - tn;  // > 0: only contributing segments are recorded
+            const float dlt = (-sigma) * dt;
+            e_dlt = pf_expf(dlt);
+            alpha = 1.0f - e_dlt;
+            rgb.x = __builtin_fmaf(tx.rgb.x * alpha, trans, rgb.x);
+            rgb.y = __builtin_fmaf(tx.rgb.y * alpha, trans, rgb.y);
+            rgb.z = __builtin_fmaf(tx.rgb.z * alpha, trans, rgb.z);
+            log_t += dlt;
+        }
+        const int slot = cache.assign(cell);
+        if (cell == kNoCell) continue;
+        bytes += segment_grad_bytes(K, tn0_id, tf0_id, n_adj);
+        segment_grads<A, F, KT>(s, colors, o, d,
+                                Seg{cell, c, nrm, r, sigma, tn0, dt, e_dlt, alpha, trans, rgb, tn_id, tf_id, tn0_id,
+                                    tf0_id, e0, n_adj, tbase},
+                                tx, g, gL, Cfin, G, exact, cache.row(slot));
+    }
+    cache.flush_all();
 }
 
 }  // namespace pf

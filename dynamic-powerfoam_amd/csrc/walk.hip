@@ -26,19 +26,25 @@ struct WalkParams {
     int *status;
     bool exact_texel;
     uint64_t *alg_bytes;  // backward per-ray byte counter (may be null)
+    // record of contributing segments, indexed by lane slot (tile * 64 +
+    // lane): the forward writes it, the recorded backward reads it
+    int4 *rec_segs;
+    int *rec_counts;
+    int rec_cap;
 };
 
 struct TileLane {
     int b, pix_i, pix_j;
     bool valid;
-    int64_t ray;
+    int64_t ray, slot;
 };
 
-__device__ __forceinline__ TileLane tile_lane(const WalkParams &p, int waves_per_block) {
+// block: the workgroup's index in the XCD-aware numbering of its tiles.
+__device__ __forceinline__ TileLane tile_lane(const WalkParams &p, int waves_per_block, int block) {
     TileLane tl;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * waves_per_block + wave;
+    const int64_t t = (int64_t)block * waves_per_block + wave;
     tl.valid = t < p.total_tiles;
     const int64_t tt = tl.valid ? t : 0;
     tl.b = (int)(tt / p.tiles_per_cam);
@@ -49,20 +55,23 @@ __device__ __forceinline__ TileLane tile_lane(const WalkParams &p, int waves_per
     tl.pix_j = tile_j * 8 + (lane >> 3);
     tl.valid = tl.valid && tl.pix_i < p.H && tl.pix_j < p.W;
     tl.ray = ((int64_t)tl.b * p.H + tl.pix_i) * p.W + tl.pix_j;
+    tl.slot = tt * 64 + lane;
     return tl;
 }
 
-template <typename A, typename F, int KT>
+template <typename A, typename F, int KT, bool REC>
 __global__ __launch_bounds__(256) void walk_fwd_kernel(WalkParams p) {
-    const TileLane tl = tile_lane(p, 4);
+    const TileLane tl = tile_lane(p, 4, xcd_block(blockIdx.x, gridDim.x));
     if (!tl.valid) return;
     const Cam cam = load_cam(p.cams + 12 * tl.b);
     const f3 d = ray_dir(cam, tl.pix_i, tl.pix_j);
     const SceneDev<A> s = scene_dev<A>(p.s);
     const MemRGB<A> colors{reinterpret_cast<const A *>(p.s.texel_rgb) + (int64_t)tl.b * p.s.texel_rgb_cam_stride};
     int *seq = p.out.seq ? p.out.seq + tl.ray * p.out.seq_cap : nullptr;
-    const RayFwd r = walk_ray_fwd<A, F, KT>(s, colors, cam.eye, d, cam.start, p.thr, p.depth_q, p.max_steps, seq,
-                                            p.out.seq_cap);
+    const SegRecord rec{REC ? p.rec_segs + tl.slot : nullptr, p.total_tiles * 64, p.rec_cap};
+    const RayFwd r = walk_ray_fwd<A, F, KT, MemRGB<A>, REC>(s, colors, cam.eye, d, cam.start, p.thr, p.depth_q,
+                                                             p.max_steps, seq, p.out.seq_cap, rec);
+    if constexpr (REC) p.rec_counts[tl.slot] = r.segs;
     float *co = p.out.color + 3 * tl.ray;
     co[0] = r.rgb.x;
     co[1] = r.rgb.y;
@@ -71,7 +80,8 @@ __global__ __launch_bounds__(256) void walk_fwd_kernel(WalkParams p) {
     if (p.out.depth) p.out.depth[tl.ray] = r.depth;
     if (p.out.visits) p.out.visits[tl.ray] = r.visits;
     if (p.out.seq_hash) p.out.seq_hash[tl.ray] = r.hash;
-    if (p.out.alg_bytes) p.out.alg_bytes[tl.ray] = r.bytes + 12 + (p.out.log_t ? 4 : 0) + (p.out.depth ? 4 : 0);
+    if (p.out.alg_bytes)
+        p.out.alg_bytes[tl.ray] = r.bytes + 12 + (p.out.log_t ? 4 : 0) + (p.out.depth ? 4 : 0) + (REC ? 4 : 0);
     if (r.status && p.out.status) atomicOr(p.out.status, r.status);
 }
 
@@ -81,20 +91,28 @@ __global__ __launch_bounds__(256) void walk_fwd_kernel(WalkParams p) {
 // catch more reuse but hold three blocks per CU and measured slower.
 constexpr int kCacheSlots = 38;
 
-// Four waves per SIMD (the LDS cache allows four blocks per CU): the
-// register budget is capped at 128, which costs a few spilled registers and
-// measured faster than three waves with no spills.
+using WaveCache = CellCache<kCacheSlots, 8>;
+constexpr int kRowsPerWave = kCacheSlots * WaveCache::kPitch;
+
+// One tile's backward by checkpoint and replay.  only_over: only the rays
+// the forward's record could not hold (count above its cap).  rows: the
+// block's LDS gradient cache (K = 8), four waves' worth.
 template <typename A, typename F, int KT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void walk_bwd_kernel(WalkParams p) {
-    const TileLane tl = tile_lane(p, 4);
+__device__ __forceinline__ void replay_tile(const WalkParams &p, const TileLane &tl, bool only_over, float *rows) {
     constexpr bool kWave = KT == 8;
     if (!kWave && !tl.valid) return;
+    const bool mine = tl.valid && (!only_over || p.rec_counts[tl.slot] > p.rec_cap);
+    if (!kWave && !mine) return;
     const int64_t ray = tl.valid ? tl.ray : 0;
-    const f3 g = p.g_color && tl.valid ? mk(p.g_color[3 * ray], p.g_color[3 * ray + 1], p.g_color[3 * ray + 2])
-                                       : mk(0, 0, 0);
-    const float gL = p.g_log_t && tl.valid ? p.g_log_t[ray] : 0.0f;
-    uint64_t bytes = 12 + (p.g_log_t ? 4 : 0);  // incoming gradients
-    const bool live = tl.valid && (g.x != 0.0f || g.y != 0.0f || g.z != 0.0f || gL != 0.0f);
+    const f3 g = p.g_color && mine ? mk(p.g_color[3 * ray], p.g_color[3 * ray + 1], p.g_color[3 * ray + 2])
+                                   : mk(0, 0, 0);
+    const float gL = p.g_log_t && mine ? p.g_log_t[ray] : 0.0f;
+    uint64_t bytes = 12 + (p.g_log_t ? 4 : 0) + (only_over ? 4 : 0);  // incoming gradients (and count)
+    const bool live = mine && (g.x != 0.0f || g.y != 0.0f || g.z != 0.0f || gL != 0.0f);
+    if (kWave && !__any(live)) {
+        if (mine && p.alg_bytes) p.alg_bytes[tl.ray] = bytes;
+        return;
+    }
     const Cam cam = load_cam(p.cams + 12 * tl.b);
     const f3 d = ray_dir(cam, tl.pix_i, tl.pix_j);
     const SceneDev<A> s = scene_dev<A>(p.s);
@@ -105,19 +123,82 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                     p.grads.texel_rgb + (int64_t)tl.b * p.grads.texel_rgb_cam_stride, p.grads.texel_height};
     int st = 0;
     if constexpr (kWave) {
-        using Cache = CellCache<kCacheSlots, 8>;
-        __shared__ float rows[4][kCacheSlots * Cache::kPitch];
-        Cache cache;
-        cache.init(rows[threadIdx.x >> 6], G);
-        st = walk_ray_bwd<A, F, KT, MemRGB<A>, Cache>(s, colors, cam.eye, d, cam.start, p.thr, p.max_steps, Cfin, g,
-                                                      gL, G, p.exact_texel, bytes, live, cache);
+        WaveCache cache;
+        cache.init(rows + (threadIdx.x >> 6) * kRowsPerWave, G);
+        st = walk_ray_bwd<A, F, KT, MemRGB<A>, WaveCache>(s, colors, cam.eye, d, cam.start, p.thr, p.max_steps, Cfin,
+                                                          g, gL, G, p.exact_texel, bytes, live, cache);
     } else if (live) {
         st = walk_ray_bwd<A, F, KT>(s, colors, cam.eye, d, cam.start, p.thr, p.max_steps, Cfin, g, gL, G,
                                     p.exact_texel, bytes);
     }
-    if (!tl.valid) return;
+    if (!mine) return;
     if (p.alg_bytes) p.alg_bytes[tl.ray] = bytes;
     if (st && p.status) atomicOr(p.status, st);
+}
+
+// One tile's backward from the forward's record of contributing segments:
+// no replay, one pass over each ray's recorded segments, with the same
+// gradient cache.  Rays whose count exceeds the cap are replay_tile's.
+template <typename A, typename F, int KT>
+__device__ __forceinline__ void record_tile(const WalkParams &p, const TileLane &tl, float *rows) {
+    constexpr bool kWave = KT == 8;
+    if (!kWave && !tl.valid) return;
+    const int count = tl.valid ? p.rec_counts[tl.slot] : 0;
+    const bool mine = tl.valid && count <= p.rec_cap;
+    if (!kWave && !mine) return;
+    const int64_t ray = tl.valid ? tl.ray : 0;
+    const f3 g = p.g_color && mine ? mk(p.g_color[3 * ray], p.g_color[3 * ray + 1], p.g_color[3 * ray + 2])
+                                   : mk(0, 0, 0);
+    const float gL = p.g_log_t && mine ? p.g_log_t[ray] : 0.0f;
+    uint64_t bytes = 12 + (p.g_log_t ? 4 : 0) + 4;  // incoming gradients, segment count
+    const bool live = mine && count > 0 && (g.x != 0.0f || g.y != 0.0f || g.z != 0.0f || gL != 0.0f);
+    if (kWave && !__any(live)) {
+        if (mine && p.alg_bytes) p.alg_bytes[tl.ray] = bytes;
+        return;
+    }
+    const Cam cam = load_cam(p.cams + 12 * tl.b);
+    const f3 d = ray_dir(cam, tl.pix_i, tl.pix_j);
+    const SceneDev<A> s = scene_dev<A>(p.s);
+    const MemRGB<A> colors{reinterpret_cast<const A *>(p.s.texel_rgb) + (int64_t)tl.b * p.s.texel_rgb_cam_stride};
+    const f3 Cfin = live ? mk(p.ck_color[3 * ray], p.ck_color[3 * ray + 1], p.ck_color[3 * ray + 2]) : mk(0, 0, 0);
+    if (live) bytes += 12;  // checkpoint colour
+    const GradDev G{p.grads.spheres, p.grads.nsigmas, p.grads.texel_sites,
+                    p.grads.texel_rgb + (int64_t)tl.b * p.grads.texel_rgb_cam_stride, p.grads.texel_height};
+    const int4 *rec = p.rec_segs + tl.slot;
+    const int64_t stride = p.total_tiles * 64;
+    if constexpr (kWave) {
+        WaveCache cache;
+        cache.init(rows + (threadIdx.x >> 6) * kRowsPerWave, G);
+        walk_rec_bwd<A, F, KT, MemRGB<A>, WaveCache>(s, colors, cam.eye, d, rec, stride, count, Cfin, g, gL, G,
+                                                     p.exact_texel, bytes, live, cache);
+    } else if (live) {
+        walk_rec_bwd<A, F, KT>(s, colors, cam.eye, d, rec, stride, count, Cfin, g, gL, G, p.exact_texel, bytes,
+                               true);
+    }
+    if (!mine) return;
+    if (p.alg_bytes) p.alg_bytes[tl.ray] = bytes;
+}
+
+// Four waves per SIMD (the LDS cache allows four blocks per CU): the
+// register budget is capped at 128, which costs a few spilled registers and
+// measured faster than three waves with no spills.
+template <typename A, typename F, int KT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void walk_bwd_kernel(WalkParams p) {
+    __shared__ float rows[KT == 8 ? 4 * kRowsPerWave : 1];
+    replay_tile<A, F, KT>(p, tile_lane(p, 4, xcd_block(blockIdx.x, gridDim.x)), false, rows);
+}
+
+// The recorded backward: grid of twice the tile blocks.  The first half
+// replays the tiles that hold a ray the record could not (most of its
+// waves find none and end at once); being dispatched first, their long
+// walks overlap the second half, which differentiates every other ray from
+// its record.
+template <typename A, typename F, int KT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void walk_bwd_rec_kernel(WalkParams p) {
+    __shared__ float rows[KT == 8 ? 4 * kRowsPerWave : 1];
+    const int nb = (int)(gridDim.x / 2);
+    if ((int)blockIdx.x < nb) replay_tile<A, F, KT>(p, tile_lane(p, 4, xcd_block(blockIdx.x, nb)), true, rows);
+    else record_tile<A, F, KT>(p, tile_lane(p, 4, xcd_block(blockIdx.x - nb, nb)), rows);
 }
 
 // Stores one walk face; the packed layout carries the neighbour too.
@@ -212,28 +293,34 @@ __global__ void start_write_kernel(const unsigned long long *ws, float *cams, in
 
 using namespace pf;
 
-template <typename A, typename F>
-static void launch_walk(const WalkParams &p, bool bwd, int K, hipStream_t st) {
+enum WalkMode { kFwd, kFwdRecord, kBwd, kBwdRecord };
+
+template <typename A, typename F, int KT> static void launch_walk_k(const WalkParams &p, WalkMode m, hipStream_t st) {
     const int64_t blocks = (p.total_tiles + 3) / 4;
     dim3 grid((unsigned)blocks), block(256);
-    if (K == 8) {
-        if (bwd) hipLaunchKernelGGL((walk_bwd_kernel<A, F, 8>), grid, block, 0, st, p);
-        else hipLaunchKernelGGL((walk_fwd_kernel<A, F, 8>), grid, block, 0, st, p);
-    } else {
-        if (bwd) hipLaunchKernelGGL((walk_bwd_kernel<A, F, 0>), grid, block, 0, st, p);
-        else hipLaunchKernelGGL((walk_fwd_kernel<A, F, 0>), grid, block, 0, st, p);
+    switch (m) {
+    case kFwd: hipLaunchKernelGGL((walk_fwd_kernel<A, F, KT, false>), grid, block, 0, st, p); break;
+    case kFwdRecord: hipLaunchKernelGGL((walk_fwd_kernel<A, F, KT, true>), grid, block, 0, st, p); break;
+    case kBwd: hipLaunchKernelGGL((walk_bwd_kernel<A, F, KT>), grid, block, 0, st, p); break;
+    case kBwdRecord: hipLaunchKernelGGL((walk_bwd_rec_kernel<A, F, KT>), dim3(2 * (unsigned)blocks), block, 0, st, p);
+        break;
     }
 }
 
-static int dispatch_walk(const WalkParams &p, bool bwd, hipStream_t st) {
+template <typename A, typename F> static void launch_walk(const WalkParams &p, WalkMode m, int K, hipStream_t st) {
+    if (K == 8) launch_walk_k<A, F, 8>(p, m, st);
+    else launch_walk_k<A, F, 0>(p, m, st);
+}
+
+static int dispatch_walk(const WalkParams &p, WalkMode m, hipStream_t st) {
     const int K = p.s.num_texel_sites;
-    if (p.s.attr_dtype == PF_F32 && p.s.face_dtype == PF_F16_PACKED) launch_walk<float, PackedF16>(p, bwd, K, st);
+    if (p.s.attr_dtype == PF_F32 && p.s.face_dtype == PF_F16_PACKED) launch_walk<float, PackedF16>(p, m, K, st);
     else if (p.s.attr_dtype == PF_F16 && p.s.face_dtype == PF_F16_PACKED)
-        launch_walk<_Float16, PackedF16>(p, bwd, K, st);
-    else if (p.s.attr_dtype == PF_F32 && p.s.face_dtype == PF_F16) launch_walk<float, _Float16>(p, bwd, K, st);
-    else if (p.s.attr_dtype == PF_F16 && p.s.face_dtype == PF_F16) launch_walk<_Float16, _Float16>(p, bwd, K, st);
-    else if (p.s.attr_dtype == PF_F32 && p.s.face_dtype == PF_F32) launch_walk<float, float>(p, bwd, K, st);
-    else if (p.s.attr_dtype == PF_F16 && p.s.face_dtype == PF_F32) launch_walk<_Float16, float>(p, bwd, K, st);
+        launch_walk<_Float16, PackedF16>(p, m, K, st);
+    else if (p.s.attr_dtype == PF_F32 && p.s.face_dtype == PF_F16) launch_walk<float, _Float16>(p, m, K, st);
+    else if (p.s.attr_dtype == PF_F16 && p.s.face_dtype == PF_F16) launch_walk<_Float16, _Float16>(p, m, K, st);
+    else if (p.s.attr_dtype == PF_F32 && p.s.face_dtype == PF_F32) launch_walk<float, float>(p, m, K, st);
+    else if (p.s.attr_dtype == PF_F16 && p.s.face_dtype == PF_F32) launch_walk<_Float16, float>(p, m, K, st);
     else return PF_ERR_DTYPE;
     return hipGetLastError() == hipSuccess ? PF_OK : PF_ERR_HIP;
 }
@@ -276,7 +363,15 @@ extern "C" int pf_walk_forward(const pf_scene *scene, const pf_camera *cams, int
     WalkParams p = make_params(scene, cams, num_cams, width, height, transmittance_threshold, max_steps);
     p.depth_q = depth_quantile;
     p.out = *out;
-    return dispatch_walk(p, false, (hipStream_t)stream);
+    if (out->record) {
+        const pf_walk_record *r = out->record;
+        if (!r->segments || !r->counts || r->cap <= 0) return PF_ERR_ARG;
+        p.rec_segs = reinterpret_cast<int4 *>(r->segments);
+        p.rec_counts = r->counts;
+        p.rec_cap = r->cap;
+        return dispatch_walk(p, kFwdRecord, (hipStream_t)stream);
+    }
+    return dispatch_walk(p, kFwd, (hipStream_t)stream);
 }
 
 extern "C" int pf_walk_backward(const pf_scene *scene, const pf_camera *cams, int32_t num_cams, int32_t width,
@@ -299,7 +394,39 @@ extern "C" int pf_walk_backward(const pf_scene *scene, const pf_camera *cams, in
     p.status = status;
     p.exact_texel = (flags & PF_BWD_EXACT_TEXEL) != 0;
     p.alg_bytes = alg_bytes;
-    return dispatch_walk(p, true, (hipStream_t)stream);
+    return dispatch_walk(p, kBwd, (hipStream_t)stream);
+}
+
+extern "C" int64_t pf_walk_record_slots(int32_t num_cams, int32_t width, int32_t height) {
+    if (num_cams <= 0 || width <= 0 || height <= 0) return 0;
+    return (int64_t)num_cams * ((height + 7) / 8) * ((width + 7) / 8) * 64;
+}
+
+extern "C" int pf_walk_backward_recorded(const pf_scene *scene, const pf_camera *cams, int32_t num_cams,
+                                         int32_t width, int32_t height, float transmittance_threshold,
+                                         int32_t max_steps, const float *color, const float *log_t,
+                                         const float *grad_color, const float *grad_log_t,
+                                         const pf_walk_record *record, const pf_grads *grads, int32_t flags,
+                                         int32_t *status, uint64_t *alg_bytes, void *stream) {
+    int rc = check_scene(scene);
+    if (rc) return rc;
+    if (!cams || num_cams <= 0 || width <= 0 || height <= 0 || !color || !grads || max_steps <= 0) return PF_ERR_ARG;
+    if (!grads->spheres || !grads->nsigmas || !grads->texel_sites || !grads->texel_rgb || !grads->texel_height)
+        return PF_ERR_ARG;
+    if (!record || !record->segments || !record->counts || record->cap <= 0) return PF_ERR_ARG;
+    WalkParams p = make_params(scene, cams, num_cams, width, height, transmittance_threshold, max_steps);
+    p.ck_color = color;
+    p.ck_log_t = log_t;
+    p.g_color = grad_color;
+    p.g_log_t = grad_log_t;
+    p.grads = *grads;
+    p.status = status;
+    p.exact_texel = (flags & PF_BWD_EXACT_TEXEL) != 0;
+    p.alg_bytes = alg_bytes;
+    p.rec_segs = reinterpret_cast<int4 *>(record->segments);
+    p.rec_counts = record->counts;
+    p.rec_cap = record->cap;
+    return dispatch_walk(p, kBwdRecord, (hipStream_t)stream);
 }
 
 extern "C" int pf_walk_faces(const float *points, const float *radii, int64_t num_cells, const int32_t *adjacency,

@@ -58,6 +58,15 @@ class PfScene(ctypes.Structure):
     ]
 
 
+class PfWalkRecord(ctypes.Structure):
+    _fields_ = [
+        ("segments", c_void_p),
+        ("counts", c_void_p),
+        ("cap", c_int32),
+        ("reserved", c_int32),
+    ]
+
+
 class PfWalkOut(ctypes.Structure):
     _fields_ = [
         ("color", c_void_p),
@@ -70,6 +79,7 @@ class PfWalkOut(ctypes.Structure):
         ("reserved", c_int32),
         ("status", c_void_p),
         ("alg_bytes", c_void_p),
+        ("record", ctypes.POINTER(PfWalkRecord)),
     ]
 
 
@@ -130,6 +140,13 @@ _SIGS = {
         c_int32,
         [ctypes.POINTER(PfScene), c_void_p, c_int32, c_int32, c_int32, c_float, c_int32, c_void_p, c_void_p,
          c_void_p, c_void_p, ctypes.POINTER(PfGrads), c_int32, c_void_p, c_void_p, c_void_p],
+    ),
+    "pf_walk_record_slots": (c_int64, [c_int32, c_int32, c_int32]),
+    "pf_walk_backward_recorded": (
+        c_int32,
+        [ctypes.POINTER(PfScene), c_void_p, c_int32, c_int32, c_int32, c_float, c_int32, c_void_p, c_void_p,
+         c_void_p, c_void_p, ctypes.POINTER(PfWalkRecord), ctypes.POINTER(PfGrads), c_int32, c_void_p, c_void_p,
+         c_void_p],
     ),
     "pf_walk_faces": (
         c_int32,

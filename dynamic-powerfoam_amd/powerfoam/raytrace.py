@@ -18,6 +18,10 @@ from .camera import cameras_to_device
 TILE_WIDTH = 8
 TILE_SIZE = TILE_WIDTH * TILE_WIDTH
 DEFAULT_MAX_STEPS = 1 << 16
+# contributing segments per ray the differentiable forward records for its
+# backward, and the most device memory one record may take
+DEFAULT_RECORD_CAP = 64
+DEFAULT_RECORD_BUDGET = 8 << 30
 
 
 def _contig(t, dtype=None):
@@ -127,9 +131,48 @@ def walk_faces(points, radii, adjacency, adjacency_offsets, dtype=torch.float16)
     return faces
 
 
+def record_slots(num_cams, width, height):
+    """Slots of a walk record: one per lane of the 8x8 tiles."""
+    return num_cams * ((height + 7) // 8) * ((width + 7) // 8) * 64
+
+
+def record_bytes(num_cams, width, height, cap):
+    return 16 * cap * record_slots(num_cams, width, height) + 4 * record_slots(num_cams, width, height)
+
+
+def new_record(num_cams, width, height, cap, device):
+    """Device buffers of a walk record holding up to cap contributing
+    segments per ray (16 bytes each, segment-major) and every ray's count."""
+    if cap <= 0:
+        raise ValueError("record cap must be positive")
+    S = record_slots(num_cams, width, height)
+    return dict(segments=torch.empty(cap, S, 4, dtype=torch.int32, device=device),
+                counts=torch.empty(S, dtype=torch.int32, device=device), cap=int(cap))
+
+
+def record_counts(record, num_cams, width, height):
+    """The record's per-ray segment counts as a (B, H, W) image.  Slot
+    tile * 64 + lane holds pixel (tile_i * 8 + lane % 8, tile_j * 8 +
+    lane // 8), tiles column-major within a camera."""
+    th, tw = (height + 7) // 8, (width + 7) // 8
+    c = record["counts"].view(num_cams, tw, th, 8, 8)  # (b, tile_j, tile_i, lane // 8, lane % 8)
+    return c.permute(0, 2, 4, 1, 3).reshape(num_cams, th * 8, tw * 8)[:, :height, :width]
+
+
+def _record_struct(record):
+    r = _lib.PfWalkRecord()
+    r.segments = _lib.ptr(record["segments"])
+    r.counts = _lib.ptr(record["counts"])
+    r.cap = record["cap"]
+    return r
+
+
 def launch_walk_forward(scene, cams_dev, width, height, transmittance_threshold=1e-3, depth_quantile=0.5,
                         max_steps=DEFAULT_MAX_STEPS, want_log_t=True, want_depth=True, want_sequences=False,
-                        seq_cap=0, want_bytes=False):
+                        seq_cap=0, want_bytes=False, record_cap=0):
+    """One walk launch over a batch of cameras.  record_cap > 0 also writes
+    the record of contributing segments that launch_walk_backward(record=)
+    differentiates without replaying the walk (out["record"])."""
     lib = _lib.load()
     B = cams_dev.shape[0]
     dev = cams_dev.device
@@ -154,17 +197,24 @@ def launch_walk_forward(scene, cams_dev, width, height, transmittance_threshold=
     out.status = _lib.ptr(status)
     alg_bytes = torch.zeros(B, height, width, dtype=torch.int64, device=dev) if want_bytes else None
     out.alg_bytes = _lib.ptr(alg_bytes)
+    record = rec = None
+    if record_cap > 0:
+        record = new_record(B, width, height, record_cap, dev)
+        rec = _record_struct(record)
+        out.record = ctypes.pointer(rec)
     rc = lib.pf_walk_forward(ctypes.byref(scene.c), _lib.ptr(cams_dev), B, width, height,
                              float(transmittance_threshold), float(depth_quantile), int(max_steps),
                              ctypes.byref(out), _lib.stream_ptr())
     _lib.check(rc, "pf_walk_forward")
     return dict(color=color, log_t=log_t, depth=depth, visits=visits, seq_hash=seq_hash, seq=seq, status=status,
-                alg_bytes=alg_bytes)
+                alg_bytes=alg_bytes, record=record)
 
 
 def launch_walk_backward(scene, cams_dev, width, height, color, log_t, grad_color, grad_log_t,
                          transmittance_threshold=1e-3, max_steps=DEFAULT_MAX_STEPS, grads=None, exact_texel=False,
-                         texel_rgb_cam_stride=0, want_bytes=False):
+                         texel_rgb_cam_stride=0, want_bytes=False, record=None):
+    """Gradients of the walk.  record: the forward's record (its "record"
+    output), read instead of replaying every ray; None replays."""
     lib = _lib.load()
     n, K = scene.num_cells, scene.K
     dev = cams_dev.device
@@ -182,12 +232,23 @@ def launch_walk_backward(scene, cams_dev, width, height, color, log_t, grad_colo
     g.texel_rgb_cam_stride = texel_rgb_cam_stride
     status = torch.zeros(1, dtype=torch.int32, device=dev)
     alg_bytes = torch.zeros(color.shape[:-1], dtype=torch.int64, device=dev) if want_bytes else None
-    rc = lib.pf_walk_backward(ctypes.byref(scene.c), _lib.ptr(cams_dev), cams_dev.shape[0], width, height,
-                              float(transmittance_threshold), int(max_steps), _lib.ptr(color), _lib.ptr(log_t),
-                              _lib.ptr(grad_color), _lib.ptr(grad_log_t), ctypes.byref(g),
-                              _lib.PF_BWD_EXACT_TEXEL if exact_texel else 0, _lib.ptr(status), _lib.ptr(alg_bytes),
-                              _lib.stream_ptr())
-    _lib.check(rc, "pf_walk_backward")
+    flags = _lib.PF_BWD_EXACT_TEXEL if exact_texel else 0
+    if record is not None:
+        if record["counts"].numel() != record_slots(cams_dev.shape[0], width, height):
+            raise ValueError("the record belongs to a walk of another size")
+        rec = _record_struct(record)
+        rc = lib.pf_walk_backward_recorded(ctypes.byref(scene.c), _lib.ptr(cams_dev), cams_dev.shape[0], width,
+                                           height, float(transmittance_threshold), int(max_steps), _lib.ptr(color),
+                                           _lib.ptr(log_t), _lib.ptr(grad_color), _lib.ptr(grad_log_t),
+                                           ctypes.byref(rec), ctypes.byref(g), flags, _lib.ptr(status),
+                                           _lib.ptr(alg_bytes), _lib.stream_ptr())
+        _lib.check(rc, "pf_walk_backward_recorded")
+    else:
+        rc = lib.pf_walk_backward(ctypes.byref(scene.c), _lib.ptr(cams_dev), cams_dev.shape[0], width, height,
+                                  float(transmittance_threshold), int(max_steps), _lib.ptr(color), _lib.ptr(log_t),
+                                  _lib.ptr(grad_color), _lib.ptr(grad_log_t), ctypes.byref(g), flags,
+                                  _lib.ptr(status), _lib.ptr(alg_bytes), _lib.stream_ptr())
+        _lib.check(rc, "pf_walk_backward")
     grads["status"] = status
     grads["alg_bytes"] = alg_bytes
     return grads
@@ -204,9 +265,11 @@ class WalkGradFn(torch.autograd.Function):
                 texel_sites, texel_rgb, texel_height, adjacency, adjacency_offsets, adjacency_diff):
         scene = WalkScene(points, radii, density, normals, texel_sites, texel_rgb, texel_height, adjacency,
                           adjacency_offsets, adjacency_diff, attr_dtype=tracer.tscalar)
+        cap = tracer.record_cap_for(cams_dev.shape[0], width, height) if any(ctx.needs_input_grad) else 0
         out = launch_walk_forward(scene, cams_dev, width, height, transmittance_threshold,
-                                  max_steps=tracer.max_steps)
+                                  max_steps=tracer.max_steps, record_cap=cap)
         ctx.scene = scene
+        ctx.record = out["record"]
         ctx.cams_dev = cams_dev
         ctx.dims = (width, height, transmittance_threshold, tracer.max_steps)
         ctx.exact_texel = tracer.exact_texel
@@ -224,12 +287,12 @@ class WalkGradFn(torch.autograd.Function):
             g_log_t = (-grad_alpha * torch.exp(log_t)).contiguous()
         gc = grad_color.contiguous() if grad_color is not None else None
         grads = launch_walk_backward(ctx.scene, ctx.cams_dev, width, height, color, log_t, gc, g_log_t, thr,
-                                     max_steps, exact_texel=ctx.exact_texel)
+                                     max_steps, exact_texel=ctx.exact_texel, record=ctx.record)
         for k in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
             t = grads[k]
             t[~t.isfinite()] = 0
         sp, ns = grads["spheres"], grads["nsigmas"]
-        del ctx.scene
+        del ctx.scene, ctx.record
         return (None, None, None, None, None, sp[:, :3], sp[:, 3], ns[:, 3], ns[:, :3], grads["texel_sites"],
                 grads["texel_rgb"], grads["texel_height"], None, None, None)
 
@@ -250,7 +313,23 @@ class RayTracer:
         self.max_steps = getattr(args, "max_walk_steps", DEFAULT_MAX_STEPS) if args is not None else DEFAULT_MAX_STEPS
         # False: the reference's texel-gradient approximations (texture.py:118-119)
         self.exact_texel = bool(getattr(args, "exact_texel_grad", False)) if args is not None else False
+        # the forward records each ray's contributing segments (up to this
+        # many) so that the backward need not replay the walk; 0 replays
+        self.record_cap = int(getattr(args, "walk_record_cap", DEFAULT_RECORD_CAP)) if args is not None \
+            else DEFAULT_RECORD_CAP
+        self.record_budget = int(getattr(args, "walk_record_budget", DEFAULT_RECORD_BUDGET)) if args is not None \
+            else DEFAULT_RECORD_BUDGET
         _lib.load()
+
+    def record_cap_for(self, num_cams, width, height):
+        """Segments per ray the differentiable forward records: record_cap,
+        lowered to fit record_budget bytes; 0 (replay) below 8."""
+        if self.record_cap <= 0:
+            return 0
+        fit = (self.record_budget - 4 * record_slots(num_cams, width, height)) // \
+            (16 * record_slots(num_cams, width, height))
+        cap = min(self.record_cap, int(fit))
+        return cap if cap >= 8 else 0
 
     def _cams(self, cameras, start_point_idx):
         if not isinstance(cameras, (list, tuple)):

@@ -81,6 +81,27 @@ typedef struct pf_scene {
                                          (view-dependent colour); 0: shared */
 } pf_scene;
 
+/* Record of each ray's contributing segments (segments with positive
+ * length after the face and texel bounds), written by pf_walk_forward for
+ * pf_walk_backward_recorded to read instead of replaying the walk.  No
+ * reference counterpart: the reference has no walk backward.
+ * Slots: one per lane of the walk's 8x8 tiles, slot = tile * 64 + lane,
+ * tiles numbered camera-major then column-major over ceil(H/8) x ceil(W/8);
+ * pf_walk_record_slots gives their number S.
+ * segments: (cap, S) 16-byte records {cell, t_near bits, t_far bits,
+ *           near face id | far face id << 16}, segment-major so that a
+ *           tile's 64 rays read one contiguous 1 KiB block per segment.
+ * counts:   (S) int32, each ray's number of contributing segments; rays
+ *           with more than cap keep only their first cap, and the backward
+ *           replays those rays instead.
+ * 16 * cap * S bytes of device memory (2.1 GB at cap 64, 1920x1080).      */
+typedef struct pf_walk_record {
+    void    *segments;
+    int32_t *counts;
+    int32_t  cap;
+    int32_t  reserved;
+} pf_walk_record;
+
 /* Per-ray outputs of the walk.  Any pointer may be NULL except color.
  * Layout (B,H,W[,3]) row-major, B = number of cameras.                    */
 typedef struct pf_walk_out {
@@ -97,6 +118,7 @@ typedef struct pf_walk_out {
     uint64_t *alg_bytes;  /* (B,H,W) algorithmic bytes the ray needed: per
                              visit sphere+CSR bounds+[n,sigma] and per face
                              index+face; per shaded segment K texels     */
+    const pf_walk_record *record;  /* NULL, or the record to write        */
 } pf_walk_out;
 
 /* Gradients produced by the walk backward; all f32, accumulated (+=) so the
@@ -154,6 +176,28 @@ int pf_walk_backward(const pf_scene *scene, const pf_camera *cams,
                      const float *grad_color, const float *grad_log_t,
                      const pf_grads *grads, int32_t flags, int32_t *status,
                      uint64_t *alg_bytes, void *stream);
+
+/* Number of record slots of a walk over num_cams cameras of width x
+ * height: num_cams * ceil(height/8) * ceil(width/8) * 64.                  */
+int64_t pf_walk_record_slots(int32_t num_cams, int32_t width, int32_t height);
+
+/* Power-cell walk, backward from the forward's record (pf_walk_out.record
+ * of the pf_walk_forward call with the same scene, cameras and
+ * threshold).  No reference counterpart, as pf_walk_backward.  Rays the
+ * record holds whole are differentiated from their recorded segments with
+ * pf_walk_backward's per-segment arithmetic and gradient cache, without
+ * reading a face row; the others are replayed as pf_walk_backward does.
+ * The results equal pf_walk_backward's up to the order of float additions.
+ * Other arguments as pf_walk_backward.                                    */
+int pf_walk_backward_recorded(const pf_scene *scene, const pf_camera *cams,
+                              int32_t num_cams, int32_t width, int32_t height,
+                              float transmittance_threshold, int32_t max_steps,
+                              const float *color, const float *log_t,
+                              const float *grad_color, const float *grad_log_t,
+                              const pf_walk_record *record,
+                              const pf_grads *grads, int32_t flags,
+                              int32_t *status, uint64_t *alg_bytes,
+                              void *stream);
 
 /* Walk face precompute (benchmark.py:295-302): for every CSR edge i->j,
  * faces[e] = half(p_j - p_i, pm_j - pm_i) with pm = 0.5(|p|^2 - r^2).

@@ -1,0 +1,152 @@
+"""The recorded backward: the forward writes each ray's contributing
+segments and the backward differentiates them without replaying the walk.
+
+Bar: the recording forward's outputs bit-equal to the plain forward's; the
+record's cells an in-order subsequence of the visited sequence; gradients
+within 1e-3 relative L2 of the oracle's, and within 1e-5 of the replay
+backward's (the two differ only in the order of float additions), with the
+record holding every ray, a few, or none (cap 1: nearly all replayed).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+from powerfoam import raytrace as rt
+from powerfoam import workloads
+
+from test_walk_gpu import _cams, _cpu_scene, _gpu_scene, _rel
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+KEYS = ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height")
+
+
+def _k(sc, K):
+    out = dict(sc)
+    for key in ("texel_sites", "texel_rgb", "texel_height"):
+        out[key] = np.ascontiguousarray(sc[key][:, :K])
+    return out
+
+
+def _run(sc, cams, cap, exact=False):
+    words = _cams(sc, cams)
+    gs, faces = _gpu_scene(sc)
+    cams_dev = torch.from_numpy(words).to(DEV)
+    W, H = cams[0].width, cams[0].height
+    plain = rt.launch_walk_forward(gs, cams_dev, W, H, want_sequences=True, seq_cap=4096)
+    fwd = rt.launch_walk_forward(gs, cams_dev, W, H, want_sequences=True, seq_cap=4096, record_cap=cap)
+    rng = np.random.default_rng(1)
+    target = torch.from_numpy(rng.uniform(0, 1, (len(cams), H, W, 3)).astype(np.float32)).to(DEV)
+    gc = 2.0 * (fwd["color"] - target)
+    gl = -0.3 * torch.exp(fwd["log_t"])
+    rec = rt.launch_walk_backward(gs, cams_dev, W, H, fwd["color"], fwd["log_t"], gc, gl, exact_texel=exact,
+                                  record=fwd["record"])
+    rep = rt.launch_walk_backward(gs, cams_dev, W, H, fwd["color"], fwd["log_t"], gc, gl, exact_texel=exact)
+    torch.cuda.synchronize()
+    for key in ("color", "log_t", "depth", "visits", "seq_hash", "seq"):
+        assert torch.equal(plain[key], fwd[key]), key
+    assert int(rec["status"].item()) == 0
+    return words, gs, faces, fwd, gc, gl, rec, rep
+
+
+@pytest.mark.parametrize("cap", [512, 6, 1])
+def test_recorded_backward_matches_cpu_and_replay(scene_textured, cap):
+    sc = scene_textured
+    cams = [workloads.front_camera(40, 40, hfov_deg=40.0)]
+    words, gs, faces, fwd, gc, gl, rec, rep = _run(sc, cams, cap)
+    counts = rt.record_counts(fwd["record"], 1, 40, 40)
+    if cap == 512:
+        assert int(counts.max().item()) <= cap
+    else:
+        assert bool((counts > cap).any()) and bool(((counts > 0) & (counts <= cap)).any())
+    cs = _cpu_scene(sc, faces)
+    c = orc.walk_forward(cs, words)
+    cg = orc.walk_backward(cs, words, c["color"], gc.cpu().numpy(), gl.cpu().numpy())
+    for key in KEYS:
+        assert _rel(rec[key].cpu().numpy(), cg[key]) < 1e-3, key
+        assert _rel(rec[key].cpu().numpy(), rep[key].cpu().numpy()) < 1e-5, key
+
+
+def test_recorded_backward_exact_texel(scene_textured):
+    words, gs, faces, fwd, gc, gl, rec, rep = _run(scene_textured, [workloads.front_camera(32, 32, hfov_deg=40.0)],
+                                                   64, exact=True)
+    for key in KEYS:
+        assert _rel(rec[key].cpu().numpy(), rep[key].cpu().numpy()) < 1e-5, key
+
+
+def test_recorded_backward_ragged_batch_and_k4(scene_textured):
+    """Three cameras of 37x29 (tiles past the image edge) and K = 4 texels
+    (the per-lane gradient path)."""
+    sc = _k(scene_textured, 4)
+    cams = workloads.sphere_cameras(np.random.default_rng(5), 3, 37, 29, radius=2.5)
+    words, gs, faces, fwd, gc, gl, rec, rep = _run(sc, cams, 64)
+    cs = _cpu_scene(sc, faces)
+    c = orc.walk_forward(cs, words)
+    cg = orc.walk_backward(cs, words, c["color"], gc.cpu().numpy(), gl.cpu().numpy())
+    for key in KEYS:
+        assert _rel(rec[key].cpu().numpy(), cg[key]) < 1e-3, key
+        assert _rel(rec[key].cpu().numpy(), rep[key].cpu().numpy()) < 1e-5, key
+
+
+def test_record_lists_contributing_cells_in_walk_order(scene_textured):
+    cams = [workloads.front_camera(24, 24, hfov_deg=40.0)]
+    words, gs, faces, fwd, gc, gl, rec, rep = _run(scene_textured, cams, 512)
+    r = fwd["record"]
+    S = r["counts"].numel()
+    seg = r["segments"].cpu().numpy()
+    cnt = r["counts"].cpu().numpy()
+    seq = fwd["seq"].cpu().numpy()[0]
+    off = scene_textured["offsets"]
+    th = (24 + 7) // 8
+    n_checked = 0
+    for slot in range(S):
+        tile, lane = divmod(slot, 64)
+        i, j = (tile % th) * 8 + lane % 8, (tile // th) * 8 + lane // 8
+        n = int(cnt[slot])
+        visited = [int(v) for v in seq[i, j] if v >= 0]
+        pos = 0
+        for k in range(n):
+            cell, tn_bits, tf_bits, ids = (int(x) for x in seg[k, slot])
+            while pos < len(visited) and visited[pos] != cell:
+                pos += 1
+            assert pos < len(visited), (slot, k, cell)
+            pos += 1
+            tn = np.int32(tn_bits).view(np.float32)
+            tf = np.int32(tf_bits).view(np.float32)
+            assert tn <= tf
+            deg = int(off[cell + 1] - off[cell])
+            lo, hi = ((ids & 0xFFFF) ^ 0x8000) - 0x8000, ids >> 16
+            assert -2 <= lo < deg and -2 <= hi < deg
+            n_checked += 1
+    assert n_checked > 100
+
+
+def test_autograd_record_and_replay_agree(scene_textured):
+    """RayTracer.forward records by default; walk_record_cap = 0 replays."""
+    import types
+
+    sc = scene_textured
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+    cam = workloads.front_camera(32, 32, hfov_deg=40.0)
+    res = []
+    for cap in (64, 0):
+        params = [t(sc[k]).requires_grad_(True) for k in
+                  ("points", "radii", "density", "normals", "texel_sites", "texel_rgb", "texel_height")]
+        adj, off = t(sc["adjacency"]).int(), t(sc["offsets"]).int()
+        faces = rt.walk_faces(params[0], params[1], adj, off)
+        tracer = rt.RayTracer(types.SimpleNamespace(walk_record_cap=cap), DEV)
+        color, alpha, _ = tracer.forward([cam], None, *params, adj, off, faces)
+        ((color ** 2).sum() + 0.5 * alpha.sum()).backward()
+        res.append([p.grad.cpu().numpy() for p in params])
+    for a, b in zip(*res):
+        assert _rel(a, b) < 1e-5
+
+
+def test_record_cap_fits_budget():
+    import types
+
+    tracer = rt.RayTracer(types.SimpleNamespace(walk_record_cap=64, walk_record_budget=1 << 30), DEV)
+    cap = tracer.record_cap_for(1, 1920, 1080)
+    assert 8 <= cap < 64 and rt.record_bytes(1, 1920, 1080, cap) <= 1 << 30
+    assert tracer.record_cap_for(64, 1920, 1080) == 0
