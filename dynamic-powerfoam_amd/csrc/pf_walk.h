@@ -267,12 +267,21 @@ __device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 
         float w;
         if constexpr (KT > 0) w = fw.w1[i];
         else w = pf_expf(-kTemp * (dot(dv, dv) * inv_r2));
-        if (!exact && w / fw.wsum < 1e-3f) continue;
+        const bool keep = exact || !(w / fw.wsum < 1e-3f);
+        if constexpr (!Sink::kUniform) {
+            if (!keep) continue;
+        }
         const float kw = k2 * w;
-        sink.rgb(i, (w * inv_ws) * dL_drgb);
+        f3 g_rgb = (w * inv_ws) * dL_drgb;
         const float gdp = dot(dL_drgb, rgbs(i, s));
         const float coef = (gdp - gdp_in) * inv_ws;
-        const f3 gs = (kw * coef) * dv;
+        f3 gs = (kw * coef) * dv;
+        if constexpr (Sink::kUniform) {
+            // every lane makes the sink calls; a skipped texel adds zeros
+            g_rgb = keep ? g_rgb : mk(0, 0, 0);
+            gs = keep ? gs : mk(0, 0, 0);
+        }
+        sink.rgb(i, g_rgb);
         sink.site(i, gs);
         dL_dq = dL_dq + (-1.0f * gs);
         if (exact) dL_dr += coef * (kw * dot(dv, dv) / r);
@@ -310,7 +319,7 @@ __device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 
             const float hdp = dL_dheight * (float)heights[i];
             const float coef = (hdp - hdp_in) * inv_ws0;
             const f3 gs = (kw * coef) * dv;
-            sink.site(i, gs);
+            sink.site_div(i, gs);
             dL_dq0 = dL_dq0 + (-1.0f * gs);
             if (exact) dL_dr += coef * (kw * dot(dv, dv) / r);
         }
@@ -503,22 +512,62 @@ struct GradDev {
     float *spheres, *nsigmas, *sites, *rgb, *height;
 };
 
+// The value of v in lane l ^ 1 (quad_perm [1, 0, 3, 2]).
+__device__ __forceinline__ float lane_xor1(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ int lane_xor1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false); }
+// The value of v in lane l ^ 8 (row_ror:8 within each row of 16 lanes).
+__device__ __forceinline__ float lane_xor8(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xF, 0xF, false));
+}
+__device__ __forceinline__ int lane_xor8(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false); }
+
+// Lane pairing of a wave whose 64 lanes all make every paired sink call:
+// lanes l and l ^ 1 (vertically adjacent pixels of the tile) that add into
+// the same cache row sum their terms first, the even lane keeping the sum
+// and the odd one zero; then lanes l and l ^ 8 (horizontally adjacent) do
+// the same, so a 2x2 quad of rays on one cell makes one LDS atomic instead
+// of four.  Each exchange moves a term to a lane of the same row, so every
+// row receives the same total.  A lane without a segment adds zeros.
+struct LanePair {
+    bool active;  // this lane has a segment
+    bool match1;  // lane l ^ 1 adds into the same row
+    bool match8;  // lane l ^ 8 adds into the same row
+    bool lead1;   // l & 1 == 0
+    bool lead8;   // l & 8 == 0
+};
+
 // Destination of one shaded segment's gradients for its own cell: the
 // sphere [c, r], [n, sigma] and the K texels' sites, colours and heights.
 // lds == nullptr adds straight into the global arrays; otherwise into the
 // cell's row of a wave's LDS accumulator (CellCache below), in the layout
 // sphere 0-3, nsigma 4-7, sites 8.., colours 8+3K.., heights 8+6K...
-// Zero contributions never reach global memory.
-struct CellSink {
+// Zero contributions never reach global memory.  PAIR: the calls of
+// sphere, nsig, rgb and site are made by all 64 lanes together and use
+// LanePair; site_div and height may be called from divergent code.
+template <bool PAIR> struct CellSink {
+    static constexpr bool kUniform = PAIR;
     float *lds;
     float *gs, *gn, *gsite, *grgb, *gh;
     int K;
-    __device__ __forceinline__ void add(int off, float *g, float v) const {
+    LanePair pr;
+    __device__ __forceinline__ void add_div(int off, float *g, float v) const {
         // zero contributions are skipped: an LDS float atomic costs per
         // active lane, and many of a segment's terms are exactly zero
         if (v == 0.0f) return;
         if (lds) lds_add(lds + off, v);
         else atomic_add(g, v);
+    }
+    __device__ __forceinline__ void add(int off, float *g, float v) const {
+        if constexpr (PAIR) {
+            v = pr.active ? v : 0.0f;
+            const float o1 = lane_xor1(v);
+            v = pr.match1 ? (pr.lead1 ? v + o1 : 0.0f) : v;
+            const float o8 = lane_xor8(v);
+            v = pr.match8 ? (pr.lead8 ? v + o8 : 0.0f) : v;
+        }
+        add_div(off, g, v);
     }
     __device__ __forceinline__ void sphere(f3 dc, float dr) const {
         add(0, gs + 0, dc.x);
@@ -542,7 +591,12 @@ struct CellSink {
         add(9 + 3 * K + 3 * i, grgb + 3 * i + 1, v.y);
         add(10 + 3 * K + 3 * i, grgb + 3 * i + 2, v.z);
     }
-    __device__ __forceinline__ void height(int i, float v) const { add(8 + 6 * K + i, gh + i, v); }
+    __device__ __forceinline__ void site_div(int i, f3 v) const {
+        add_div(8 + 3 * i, gsite + 3 * i, v.x);
+        add_div(9 + 3 * i, gsite + 3 * i + 1, v.y);
+        add_div(10 + 3 * i, gsite + 3 * i + 2, v.z);
+    }
+    __device__ __forceinline__ void height(int i, float v) const { add_div(8 + 6 * K + i, gh + i, v); }
 };
 
 // Per-lane policy: every lane sends its gradients straight to global memory
@@ -678,10 +732,11 @@ __device__ __forceinline__ uint64_t segment_grad_bytes(int K, int tn0_id, int tf
 // routed to the sphere, a power face or the texel plane.  The cell's own
 // gradients go to row (an LDS row of the wave's cache) or, row == nullptr,
 // to global memory; neighbour spheres get global atomics.
-template <typename A, typename F, int KT, typename RGB>
+template <typename A, typename F, int KT, typename RGB, bool PAIR = false>
 __device__ __forceinline__ void segment_grads(const SceneDev<A> &s, const RGB &colors, f3 o, f3 d, const Seg &q,
                                               const TexFwdT<KT> &tx, f3 g, float gL, f3 Cfin, const GradDev &G,
-                                              bool exact, float *row) {
+                                              bool exact, float *row,
+                                              LanePair pr = LanePair{true, false, false, false, false}) {
     const int K = KT > 0 ? KT : s.K;
     const int cell = q.cell, n_adj = q.n_adj;
     const f3 c = q.c;
@@ -699,13 +754,14 @@ __device__ __forceinline__ void segment_grads(const SceneDev<A> &s, const RGB &c
     else if (q.tf_id == n_adj) dL_dts = dL_dtf;
 
     const int64_t tbase = q.tbase;
-    const CellSink sink{row,
-                        G.spheres + 4 * (int64_t)cell,
-                        G.nsigmas + 4 * (int64_t)cell,
-                        G.sites + 3 * tbase,
-                        G.rgb + 3 * tbase,
-                        G.height + tbase,
-                        K};
+    const CellSink<PAIR> sink{row,
+                              G.spheres + 4 * (int64_t)cell,
+                              G.nsigmas + 4 * (int64_t)cell,
+                              G.sites + 3 * tbase,
+                              G.rgb + 3 * tbase,
+                              G.height + tbase,
+                              K,
+                              pr};
     const auto rgbs = colors.cell(cell, K);
     f3 dL_dc, dL_dn;
     float dL_dtn0, dL_dr = 0.0f;
@@ -922,12 +978,28 @@ __device__ __forceinline__ void walk_rec_bwd(const SceneDev<A> &s, const RGB &co
             log_t += dlt;
         }
         const int slot = cache.assign(cell);
-        if (cell == kNoCell) continue;
-        bytes += segment_grad_bytes(K, tn0_id, tf0_id, n_adj);
-        segment_grads<A, F, KT>(s, colors, o, d,
-                                Seg{cell, c, nrm, r, sigma, tn0, dt, e_dlt, alpha, trans, rgb, tn_id, tf_id, tn0_id,
-                                    tf0_id, e0, n_adj, tbase},
-                                tx, g, gL, Cfin, G, exact, cache.row(slot));
+        if constexpr (Cache::kWave) {
+            // all 64 lanes make the gradient calls, so that pairs of lanes
+            // on one cell can sum their terms before the LDS atomics; a
+            // lane without a segment computes on cell 0 and adds zeros
+            const int s1 = lane_xor1(slot), s8 = lane_xor8(slot);
+            const int lane = __lane_id();
+            const LanePair pr{cell != kNoCell, slot >= 0 && s1 == slot, slot >= 0 && s8 == slot, (lane & 1) == 0,
+                              (lane & 8) == 0};
+            if (cell != kNoCell) bytes += segment_grad_bytes(K, tn0_id, tf0_id, n_adj);
+            else tn0_id = tf0_id = -1;  // the sphere bounds, so no neighbour is touched
+            segment_grads<A, F, KT, RGB, true>(s, colors, o, d,
+                                               Seg{cell != kNoCell ? cell : 0, c, nrm, r, sigma, tn0, dt, e_dlt,
+                                                   alpha, trans, rgb, tn_id, tf_id, tn0_id, tf0_id, e0, n_adj, tbase},
+                                               tx, g, gL, Cfin, G, exact, cache.row(slot), pr);
+        } else {
+            if (cell == kNoCell) continue;
+            bytes += segment_grad_bytes(K, tn0_id, tf0_id, n_adj);
+            segment_grads<A, F, KT>(s, colors, o, d,
+                                    Seg{cell, c, nrm, r, sigma, tn0, dt, e_dlt, alpha, trans, rgb, tn_id, tf_id,
+                                        tn0_id, tf0_id, e0, n_adj, tbase},
+                                    tx, g, gL, Cfin, G, exact, cache.row(slot));
+        }
     }
     cache.flush_all();
 }
