@@ -175,9 +175,10 @@ def main():
     segs = None
     if f["record"] is not None:
         cnt = rt.record_counts(f["record"], 1, W, H).double()
-        segs = dict(cap=args.record_cap, mean=round(cnt.mean().item(), 2),
+        cap = f["record"]["cap"]
+        segs = dict(cap=cap, mean=round(cnt.mean().item(), 2),
                     p99=float(torch.quantile(cnt.flatten()[::7], 0.99).item()), max=int(cnt.max().item()),
-                    replayed_rays=int((cnt > args.record_cap).sum().item()))
+                    replayed_rays=int((cnt > cap).sum().item()))
         log(f"record: contributing segments per ray {segs}")
     fwd_bytes = float(f["alg_bytes"].sum().item())
     bwd_bytes = float(b["alg_bytes"].sum().item())

@@ -398,14 +398,22 @@ struct RayFwd {
     int segs;  // contributing segments (recorded or not)
 };
 
-// A ray's record of contributing segments: segment k goes to rec[k * stride]
-// as {cell, face-bounded t_near bits, face-bounded t_far bits,
+// A ray's record of contributing segments: segment k of the ray in slot
+// `slot` of a record over S slots is entry rec_index(k, slot, S), as
+// {cell, face-bounded t_near bits, face-bounded t_far bits,
 // near face id (low 16 bits) | far face id (high 16 bits)}; ids index the
 // cell's CSR row, -1 for the sphere, -2 for a sphere bound behind the
-// origin.  Segments past cap are counted but not stored.
+// origin.  Segments past cap are counted but not stored.  Eight
+// consecutive segments of one ray share a 128-byte line (the forward's
+// scattered writes fill lines), and a tile's 64 rays read 64 lines that
+// serve their next seven segments too.  cap is a multiple of 8.
+constexpr int kRecChunk = 8;
+__device__ __forceinline__ int64_t rec_index(int k, int64_t slot, int64_t S) {
+    return ((int64_t)(k / kRecChunk) * S + slot) * kRecChunk + (k % kRecChunk);
+}
 struct SegRecord {
-    int4 *rec;
-    int64_t stride;
+    int4 *rec;  // the record's first entry
+    int64_t slot, S;
     int cap;
 };
 constexpr uint64_t kSegRecordBytes = 16;
@@ -420,7 +428,7 @@ __device__ __forceinline__ int4 pack_segment(int cell, float tn, float tf, int t
 template <typename A, typename F, int KT, typename RGB, bool REC = false>
 __device__ __forceinline__ RayFwd walk_ray_fwd(const SceneDev<A> &s, const RGB &colors, f3 o, f3 d, int start,
                                                float thr, float depth_q, int max_steps, int *seq, int seq_cap,
-                                               SegRecord rec = SegRecord{nullptr, 0, 0}) {
+                                               SegRecord rec = SegRecord{nullptr, 0, 0, 0}) {
     constexpr uint64_t kVisitBytes = WalkBytes<A, F>::visit, kFaceBytes = WalkBytes<A, F>::face,
                        kTexelBytes = WalkBytes<A, F>::texel;
     const int K = KT > 0 ? KT : s.K;
@@ -446,7 +454,8 @@ __device__ __forceinline__ RayFwd walk_ray_fwd(const SceneDev<A> &s, const RGB &
         SphereHit sh = ray_sphere(o, d, c, r);
         if (length(c - o) < 4.0f * r) sh.hit = false;
         float tn = sh.tn, tf = sh.tf;
-        // the bounding faces, tracked as walk_ray_bwd's replay tracks them
+        // the bounding faces: the near one tracked as walk_ray_bwd's replay
+        // tracks it, the far one from the exit face after the scan
         int tn_id = 0, tf_id = 0;
         if constexpr (REC) {
             tn_id = tn < 1e-8f ? -2 : -1;
@@ -463,7 +472,6 @@ __device__ __forceinline__ RayFwd walk_ray_fwd(const SceneDev<A> &s, const RGB &
             ray_pface_diff(o, d, fn, fw, t, dp);
             if (dp >= 0.0f && t < pt_far) { exit_e = e; pt_far = t; }
             if constexpr (REC) {
-                if (dp >= 0.0f && t < tf) tf_id = e - e0;
                 if (dp < 0.0f && t > tn) tn_id = e - e0;
             }
             tf = dp >= 0.0f ? fmin_(t, tf) : tf;
@@ -471,6 +479,14 @@ __device__ __forceinline__ RayFwd walk_ray_fwd(const SceneDev<A> &s, const RGB &
         });
         const int next = exit_e >= 0 ? face_neighbour<F>(s.faces, s.adj, exit_e) : kNoCell;
         if (!sh.hit || tn > tf || sigma < 1e-3f) { prim = next; continue; }
+        if constexpr (REC) {
+            // the face bounding t_far is the exit face when the exit comes
+            // before the sphere's far bound (both are the first face at the
+            // minimum t of the faces ahead); the exit search starts at 1e10,
+            // so this holds for sphere bounds up to 1e10 (larger ones send
+            // the ray to the replay, below)
+            if (exit_e >= 0 && pt_far < sh.tf) tf_id = exit_e - e0;
+        }
 
         const int64_t tbase = (int64_t)prim * K;
         out.bytes += (uint64_t)K * kTexelBytes;
@@ -484,8 +500,9 @@ __device__ __forceinline__ RayFwd walk_ray_fwd(const SceneDev<A> &s, const RGB &
         const float dt = tf - tn;
         if (dt > 0.0f) {
             if constexpr (REC) {
+                if (!(sh.tf <= 1e10f)) out.segs = out.segs > rec.cap ? out.segs : rec.cap;
                 if (out.segs < rec.cap) {
-                    rec.rec[out.segs * rec.stride] = pack_segment(cell, tn_face, tf_face, tn_id, tf_id);
+                    rec.rec[rec_index(out.segs, rec.slot, rec.S)] = pack_segment(cell, tn_face, tf_face, tn_id, tf_id);
                     out.bytes += kSegRecordBytes;
                 }
                 ++out.segs;
@@ -923,16 +940,27 @@ __device__ __forceinline__ int walk_ray_bwd(const SceneDev<A> &s, const RGB &col
 // transmittance are bit-identical; no face row is read.  count is the
 // ray's number of contributing segments, all of them in the record (a ray
 // whose count exceeds the record's cap goes through walk_ray_bwd instead).
+// A recorded cell outside the scene (a record of another walk) is skipped
+// and reported as PF_STATUS_BAD_RECORD, which the function returns.
 template <typename A, typename F, int KT, typename RGB, typename Cache = LaneCache>
-__device__ __forceinline__ void walk_rec_bwd(const SceneDev<A> &s, const RGB &colors, f3 o, f3 d, const int4 *rec,
-                                             int64_t stride, int count, f3 Cfin, f3 g, float gL, const GradDev &G,
+__device__ __forceinline__ int walk_rec_bwd(const SceneDev<A> &s, const RGB &colors, f3 o, f3 d, const int4 *rec,
+                                             int64_t slot, int64_t S, int count, f3 Cfin, f3 g, float gL, const GradDev &G,
                                              bool exact, uint64_t &bytes, bool live, Cache cache = Cache{}) {
     constexpr uint64_t kVisitBytes = WalkBytes<A, F>::visit, kTexelBytes = WalkBytes<A, F>::texel;
     const int K = KT > 0 ? KT : s.K;
     f3 rgb = mk(0, 0, 0);
     float log_t = 0.0f;
+    int status = 0;
     for (int k = 0;; ++k) {
-        const bool act = live && k < count;
+        bool act = live && k < count;
+        int4 q = make_int4(0, 0, 0, 0);
+        if (act) {
+            q = rec[rec_index(k, slot, S)];
+            if ((unsigned)q.x >= (unsigned)s.N) {
+                status = PF_STATUS_BAD_RECORD;
+                live = act = false;
+            }
+        }
         if constexpr (Cache::kWave) {
             if (!__any(act)) break;
         } else {
@@ -945,7 +973,6 @@ __device__ __forceinline__ void walk_rec_bwd(const SceneDev<A> &s, const RGB &co
         int64_t tbase = 0;
         TexFwdT<KT> tx{};
         if (act) {
-            const int4 q = rec[k * stride];
             cell = q.x;
             float tn = __int_as_float(q.y), tf = __int_as_float(q.z);
             tn_id = (int)(int16_t)(q.w & 0xffff);
@@ -1002,6 +1029,7 @@ __device__ __forceinline__ void walk_rec_bwd(const SceneDev<A> &s, const RGB &co
         }
     }
     cache.flush_all();
+    return status;
 }
 
 }  // namespace pf

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(256) void walk_fwd_kernel(WalkParams p) {
     const SceneDev<A> s = scene_dev<A>(p.s);
     const MemRGB<A> colors{reinterpret_cast<const A *>(p.s.texel_rgb) + (int64_t)tl.b * p.s.texel_rgb_cam_stride};
     int *seq = p.out.seq ? p.out.seq + tl.ray * p.out.seq_cap : nullptr;
-    const SegRecord rec{REC ? p.rec_segs + tl.slot : nullptr, p.total_tiles * 64, p.rec_cap};
+    const SegRecord rec{p.rec_segs, tl.slot, p.total_tiles * 64, p.rec_cap};
     const RayFwd r = walk_ray_fwd<A, F, KT, MemRGB<A>, REC>(s, colors, cam.eye, d, cam.start, p.thr, p.depth_q,
                                                              p.max_steps, seq, p.out.seq_cap, rec);
     if constexpr (REC) p.rec_counts[tl.slot] = r.segs;
@@ -164,19 +164,20 @@ __device__ __forceinline__ void record_tile(const WalkParams &p, const TileLane 
     if (live) bytes += 12;  // checkpoint colour
     const GradDev G{p.grads.spheres, p.grads.nsigmas, p.grads.texel_sites,
                     p.grads.texel_rgb + (int64_t)tl.b * p.grads.texel_rgb_cam_stride, p.grads.texel_height};
-    const int4 *rec = p.rec_segs + tl.slot;
-    const int64_t stride = p.total_tiles * 64;
+    const int64_t S = p.total_tiles * 64;
+    int st = 0;
     if constexpr (kWave) {
         WaveCache cache;
         cache.init(rows + (threadIdx.x >> 6) * kRowsPerWave, G);
-        walk_rec_bwd<A, F, KT, MemRGB<A>, WaveCache>(s, colors, cam.eye, d, rec, stride, count, Cfin, g, gL, G,
-                                                     p.exact_texel, bytes, live, cache);
+        st = walk_rec_bwd<A, F, KT, MemRGB<A>, WaveCache>(s, colors, cam.eye, d, p.rec_segs, tl.slot, S, count, Cfin,
+                                                          g, gL, G, p.exact_texel, bytes, live, cache);
     } else if (live) {
-        walk_rec_bwd<A, F, KT>(s, colors, cam.eye, d, rec, stride, count, Cfin, g, gL, G, p.exact_texel, bytes,
-                               true);
+        st = walk_rec_bwd<A, F, KT>(s, colors, cam.eye, d, p.rec_segs, tl.slot, S, count, Cfin, g, gL, G,
+                                    p.exact_texel, bytes, true);
     }
     if (!mine) return;
     if (p.alg_bytes) p.alg_bytes[tl.ray] = bytes;
+    if (st && p.status) atomicOr(p.status, st);
 }
 
 // Four waves per SIMD (the LDS cache allows four blocks per CU): the
@@ -365,7 +366,7 @@ extern "C" int pf_walk_forward(const pf_scene *scene, const pf_camera *cams, int
     p.out = *out;
     if (out->record) {
         const pf_walk_record *r = out->record;
-        if (!r->segments || !r->counts || r->cap <= 0) return PF_ERR_ARG;
+        if (!r->segments || !r->counts || r->cap <= 0 || r->cap % kRecChunk) return PF_ERR_ARG;
         p.rec_segs = reinterpret_cast<int4 *>(r->segments);
         p.rec_counts = r->counts;
         p.rec_cap = r->cap;
@@ -413,7 +414,8 @@ extern "C" int pf_walk_backward_recorded(const pf_scene *scene, const pf_camera 
     if (!cams || num_cams <= 0 || width <= 0 || height <= 0 || !color || !grads || max_steps <= 0) return PF_ERR_ARG;
     if (!grads->spheres || !grads->nsigmas || !grads->texel_sites || !grads->texel_rgb || !grads->texel_height)
         return PF_ERR_ARG;
-    if (!record || !record->segments || !record->counts || record->cap <= 0) return PF_ERR_ARG;
+    if (!record || !record->segments || !record->counts || record->cap <= 0 || record->cap % kRecChunk)
+        return PF_ERR_ARG;
     WalkParams p = make_params(scene, cams, num_cams, width, height, transmittance_threshold, max_steps);
     p.ck_color = color;
     p.ck_log_t = log_t;

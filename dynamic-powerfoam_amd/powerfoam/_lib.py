@@ -16,7 +16,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpf_hip.so")
 
 PF_OK, PF_ERR_ARG, PF_ERR_DTYPE, PF_ERR_HIP, PF_ERR_OVERFLOW = 0, 1, 2, 3, 4
-PF_STATUS_STEP_CAP, PF_STATUS_LIST_CAP, PF_STATUS_CYCLE = 1, 2, 4
+PF_STATUS_STEP_CAP, PF_STATUS_LIST_CAP, PF_STATUS_CYCLE, PF_STATUS_BAD_RECORD = 1, 2, 4, 8
 PF_F32, PF_F16, PF_F16_PACKED = 0, 1, 2
 PF_BWD_EXACT_TEXEL = 1
 PF_SERVE_SH_IN_WALK = 1

@@ -140,14 +140,26 @@ def record_bytes(num_cams, width, height, cap):
     return 16 * cap * record_slots(num_cams, width, height) + 4 * record_slots(num_cams, width, height)
 
 
+RECORD_CHUNK = 8  # segments of one ray per 128-byte line; caps are multiples
+
+
 def new_record(num_cams, width, height, cap, device):
     """Device buffers of a walk record holding up to cap contributing
-    segments per ray (16 bytes each, segment-major) and every ray's count."""
+    segments per ray (16 bytes each; cap rounded up to a multiple of 8) and
+    every ray's count.  Segment k of slot s is row ((k // 8) * S + s) * 8 +
+    k % 8 of "segments" viewed as (cap * S, 4)."""
     if cap <= 0:
         raise ValueError("record cap must be positive")
+    cap = -(-int(cap) // RECORD_CHUNK) * RECORD_CHUNK
     S = record_slots(num_cams, width, height)
-    return dict(segments=torch.empty(cap, S, 4, dtype=torch.int32, device=device),
-                counts=torch.empty(S, dtype=torch.int32, device=device), cap=int(cap))
+    return dict(segments=torch.empty(cap * S, 4, dtype=torch.int32, device=device),
+                counts=torch.empty(S, dtype=torch.int32, device=device), cap=cap)
+
+
+def record_segment(record, k, slot):
+    """Row of segment k of slot `slot` in record["segments"]."""
+    S = record["counts"].numel()
+    return ((k // RECORD_CHUNK) * S + slot) * RECORD_CHUNK + k % RECORD_CHUNK
 
 
 def record_counts(record, num_cams, width, height):
@@ -328,7 +340,7 @@ class RayTracer:
             return 0
         fit = (self.record_budget - 4 * record_slots(num_cams, width, height)) // \
             (16 * record_slots(num_cams, width, height))
-        cap = min(self.record_cap, int(fit))
+        cap = min(self.record_cap, int(fit)) // RECORD_CHUNK * RECORD_CHUNK
         return cap if cap >= 8 else 0
 
     def _cams(self, cameras, start_point_idx):

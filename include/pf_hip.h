@@ -36,6 +36,9 @@ extern "C" {
                                      it held before, so it would repeat
                                      forever (raytrace.py:163 never ends on
                                      such a ray); the ray stops there      */
+#define PF_STATUS_BAD_RECORD  8   /* a walk record named a cell outside the
+                                     scene (a record of another walk); the
+                                     segment was skipped                    */
 
 /* ---- dtypes ------------------------------------------------------------ */
 #define PF_F32 0
@@ -88,12 +91,14 @@ typedef struct pf_scene {
  * Slots: one per lane of the walk's 8x8 tiles, slot = tile * 64 + lane,
  * tiles numbered camera-major then column-major over ceil(H/8) x ceil(W/8);
  * pf_walk_record_slots gives their number S.
- * segments: (cap, S) 16-byte records {cell, t_near bits, t_far bits,
- *           near face id | far face id << 16}, segment-major so that a
- *           tile's 64 rays read one contiguous 1 KiB block per segment.
+ * segments: cap * S 16-byte records {cell, t_near bits, t_far bits,
+ *           near face id | far face id << 16}; segment k of slot s is
+ *           record ((k / 8) * S + s) * 8 + k % 8, so eight consecutive
+ *           segments of a ray fill one 128-byte line.
  * counts:   (S) int32, each ray's number of contributing segments; rays
  *           with more than cap keep only their first cap, and the backward
  *           replays those rays instead.
+ * cap:      a positive multiple of 8.
  * 16 * cap * S bytes of device memory (2.1 GB at cap 64, 1920x1080).      */
 typedef struct pf_walk_record {
     void    *segments;

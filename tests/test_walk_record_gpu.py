@@ -5,7 +5,7 @@ Bar: the recording forward's outputs bit-equal to the plain forward's; the
 record's cells an in-order subsequence of the visited sequence; gradients
 within 1e-3 relative L2 of the oracle's, and within 1e-5 of the replay
 backward's (the two differ only in the order of float additions), with the
-record holding every ray, a few, or none (cap 1: nearly all replayed).
+record holding every ray or only some (cap 8: many replayed).
 """
 import numpy as np
 import pytest
@@ -50,13 +50,23 @@ def _run(sc, cams, cap, exact=False):
     return words, gs, faces, fwd, gc, gl, rec, rep
 
 
-@pytest.mark.parametrize("cap", [512, 6, 1])
-def test_recorded_backward_matches_cpu_and_replay(scene_textured, cap):
+@pytest.mark.parametrize("mixed", [False, True])
+def test_recorded_backward_matches_cpu_and_replay(scene_textured, mixed):
+    """mixed: a cap below the longest rays, so that the launch replays some
+    rays and reads the others from the record."""
     sc = scene_textured
     cams = [workloads.front_camera(40, 40, hfov_deg=40.0)]
+    cap = 512
+    if mixed:
+        words = _cams(sc, cams)
+        gs, _ = _gpu_scene(sc)
+        probe = rt.launch_walk_forward(gs, torch.from_numpy(words).to(DEV), 40, 40, record_cap=512)
+        longest = int(probe["record"]["counts"].max().item())
+        assert longest > 8
+        cap = max(8, longest // 2 // 8 * 8)
     words, gs, faces, fwd, gc, gl, rec, rep = _run(sc, cams, cap)
     counts = rt.record_counts(fwd["record"], 1, 40, 40)
-    if cap == 512:
+    if not mixed:
         assert int(counts.max().item()) <= cap
     else:
         assert bool((counts > cap).any()) and bool(((counts > 0) & (counts <= cap)).any())
@@ -96,6 +106,7 @@ def test_record_lists_contributing_cells_in_walk_order(scene_textured):
     S = r["counts"].numel()
     seg = r["segments"].cpu().numpy()
     cnt = r["counts"].cpu().numpy()
+    assert r["cap"] == 512 and seg.shape == (512 * S, 4)
     seq = fwd["seq"].cpu().numpy()[0]
     off = scene_textured["offsets"]
     th = (24 + 7) // 8
@@ -107,7 +118,7 @@ def test_record_lists_contributing_cells_in_walk_order(scene_textured):
         visited = [int(v) for v in seq[i, j] if v >= 0]
         pos = 0
         for k in range(n):
-            cell, tn_bits, tf_bits, ids = (int(x) for x in seg[k, slot])
+            cell, tn_bits, tf_bits, ids = (int(x) for x in seg[rt.record_segment(r, k, slot)])
             while pos < len(visited) and visited[pos] != cell:
                 pos += 1
             assert pos < len(visited), (slot, k, cell)
@@ -148,5 +159,26 @@ def test_record_cap_fits_budget():
 
     tracer = rt.RayTracer(types.SimpleNamespace(walk_record_cap=64, walk_record_budget=1 << 30), DEV)
     cap = tracer.record_cap_for(1, 1920, 1080)
-    assert 8 <= cap < 64 and rt.record_bytes(1, 1920, 1080, cap) <= 1 << 30
+    assert 8 <= cap < 64 and cap % 8 == 0 and rt.record_bytes(1, 1920, 1080, cap) <= 1 << 30
     assert tracer.record_cap_for(64, 1920, 1080) == 0
+
+
+def test_record_of_another_walk_is_reported_not_read(scene_textured):
+    """A recorded cell outside the scene is caught by the backward's bounds
+    check: PF_STATUS_BAD_RECORD, the ray's remaining segments skipped."""
+    from powerfoam import _lib
+
+    sc = scene_textured
+    cams = [workloads.front_camera(16, 16, hfov_deg=40.0)]
+    words = _cams(sc, cams)
+    gs, _ = _gpu_scene(sc)
+    cams_dev = torch.from_numpy(words).to(DEV)
+    fwd = rt.launch_walk_forward(gs, cams_dev, 16, 16, record_cap=64)
+    r = fwd["record"]
+    slot = int(torch.nonzero(r["counts"] > 0)[0, 0].item())
+    r["segments"][rt.record_segment(r, 0, slot), 0] = sc["points"].shape[0]  # one past the last cell
+    gc = torch.ones_like(fwd["color"])
+    g = rt.launch_walk_backward(gs, cams_dev, 16, 16, fwd["color"], fwd["log_t"], gc, None, record=r)
+    torch.cuda.synchronize()
+    assert int(g["status"].item()) & _lib.PF_STATUS_BAD_RECORD
+    assert all(bool(g[k].isfinite().all()) for k in KEYS)

@@ -1,0 +1,43 @@
+"""Host side of the walk record: slot count, allocation, the chunked
+segment index and the slot -> pixel map (no GPU)."""
+import numpy as np
+import pytest
+import torch
+
+from powerfoam import raytrace as rt
+
+
+def test_slots_cover_whole_tiles():
+    assert rt.record_slots(1, 1920, 1080) == 240 * 135 * 64
+    assert rt.record_slots(3, 37, 29) == 3 * 5 * 4 * 64
+
+
+def test_cap_rounds_up_to_the_chunk():
+    r = rt.new_record(1, 16, 16, 13, "cpu")
+    assert r["cap"] == 16 and r["segments"].shape == (16 * 4 * 64, 4)
+    with pytest.raises(ValueError):
+        rt.new_record(1, 16, 16, 0, "cpu")
+
+
+def test_segment_index_is_a_bijection_with_chunks_of_eight():
+    r = rt.new_record(2, 24, 16, 24, "cpu")
+    S = r["counts"].numel()
+    rows = np.array([[rt.record_segment(r, k, s) for s in range(S)] for k in range(24)])
+    assert sorted(rows.ravel().tolist()) == list(range(24 * S))
+    # eight consecutive segments of a slot are one 128-byte line
+    assert np.array_equal(rows[8:16, 5], rows[8, 5] + np.arange(8)) and rows[8, 5] % 8 == 0
+
+
+def test_counts_image_follows_the_tile_lane_map():
+    B, W, H = 2, 20, 13
+    r = rt.new_record(B, W, H, 8, "cpu")
+    S = r["counts"].numel()
+    r["counts"].copy_(torch.arange(S, dtype=torch.int32))
+    img = rt.record_counts(r, B, W, H).numpy()
+    th, tw = (H + 7) // 8, (W + 7) // 8
+    for b in range(B):
+        for i in range(H):
+            for j in range(W):
+                tile = b * th * tw + (j // 8) * th + i // 8
+                lane = (i % 8) + 8 * (j % 8)
+                assert img[b, i, j] == tile * 64 + lane
