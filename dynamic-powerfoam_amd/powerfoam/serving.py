@@ -40,7 +40,7 @@ import torch
 
 from . import _lib
 from .camera import cameras_to_device, pack_cameras
-from .raytrace import WalkScene, launch_walk_backward, launch_walk_forward, start_cells_device
+from .raytrace import DEFAULT_RECORD_CAP, WalkScene, launch_walk_backward, launch_walk_forward, start_cells_device
 from .shading import sh_backward, sh_forward
 
 TILE = 8
@@ -444,13 +444,16 @@ class PlainRenderer:
             scene = _SceneView(c, w.num_cells, w.K, rgb)
         else:
             scene = w
-        out = launch_walk_forward(scene, cam, r.width, r.height, self.thr, want_depth=False)
+        # a differentiable render records its contributing segments, as
+        # RayTracer.forward does, and the backward reads them
+        cap = DEFAULT_RECORD_CAP if r.mode == "forward_backward" else 0
+        out = launch_walk_forward(scene, cam, r.width, r.height, self.thr, want_depth=False, record_cap=cap)
         r.color, r.log_t = out["color"][0], out["log_t"][0]
         if r.mode == "forward_backward":
             tgt = r.target.to(dev, torch.float32)
             gc = 2.0 * (out["color"] - tgt[None])
             r.grads = launch_walk_backward(scene, cam, r.width, r.height, out["color"], out["log_t"], gc, None,
-                                           self.thr)
+                                           self.thr, record=out["record"])
             if sc.sh is not None:
                 sh_backward(sc.sh, 3, w.texel_sites, cam, r.grads["texel_rgb"][None])
         return r
