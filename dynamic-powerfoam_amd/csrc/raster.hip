@@ -278,24 +278,42 @@ __global__ void raster_faces_kernel(const float4 *spheres, int64_t N, const int 
 }
 
 // ----------------------------------------------------------- texel fwd --
-struct RTex { float t0, dp0, height, wsum0, t_surf, dp, wsum; f3 rgb; };
+// w0 / w1: the texel weights at the plane entry q0 and at the clipped entry
+// q, kept for the backward (at most 8 texels on the raster path).
+struct RTex {
+    float t0, dp0, height, wsum0, t_surf, dp, wsum;
+    f3 rgb;
+    float w0[8], w1[8];
+};
 
 // powerfoam/texture.py:10-71 (raster variant: weights divide by r*r; if the
 // ray misses, everything is zero).  tex: [sites 3K | rgb 3K | height K].
+// KEEP: also keep the weights (K <= 8), for the backward.
+template <bool KEEP>
 __device__ __forceinline__ RTex texel_fwd_raster(f3 o, f3 d, float t_near, f3 c, f3 n, float r, bool hit,
                                                  const float *tex, int K) {
-    RTex out{0, 0, 0, 0, 0, 0, 0, mk(0, 0, 0)};
+    RTex out{0, 0, 0, 0, 0, 0, 0, mk(0, 0, 0), {}, {}};
     if (!hit) return out;
     ray_plane(o, d, c, n, 0.0f, out.t0, out.dp0);
     const float tq0 = out.dp0 >= 0.0f ? t_near : fmax_(t_near, out.t0);
     const f3 q0 = ray_at(o, tq0, d);
     const float r2 = r * r;
     float hs = 0.0f, ws = 0.0f;
-    for (int i = 0; i < K; ++i) {
+    auto height_term = [&](int i) {
         const f3 dv = q0 - mk(tex[3 * i], tex[3 * i + 1], tex[3 * i + 2]);
         const float w = pf_expf(-kTemp * (dot(dv, dv) / r2));
         hs = __builtin_fmaf(w, tex[6 * K + i], hs);
         ws = ws + w;
+        return w;
+    };
+    if constexpr (KEEP) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (i >= K) break;
+            out.w0[i] = height_term(i);
+        }
+    } else {
+        for (int i = 0; i < K; ++i) height_term(i);
     }
     ws = fmax_(ws, 1e-20f);
     out.wsum0 = ws;
@@ -305,11 +323,21 @@ __device__ __forceinline__ RTex texel_fwd_raster(f3 o, f3 d, float t_near, f3 c,
     const f3 q = ray_at(o, tq, d);
     f3 acc = mk(0, 0, 0);
     float ws2 = 0.0f;
-    for (int i = 0; i < K; ++i) {
+    auto colour_term = [&](int i) {
         const f3 dv = q - mk(tex[3 * i], tex[3 * i + 1], tex[3 * i + 2]);
         const float w = pf_expf(-kTemp * (dot(dv, dv) / r2));
         acc = macc(acc, w, mk(tex[3 * K + 3 * i], tex[3 * K + 3 * i + 1], tex[3 * K + 3 * i + 2]));
         ws2 = ws2 + w;
+        return w;
+    };
+    if constexpr (KEEP) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (i >= K) break;
+            out.w1[i] = colour_term(i);
+        }
+    } else {
+        for (int i = 0; i < K; ++i) colour_term(i);
     }
     ws2 = fmax_(ws2, 1e-20f);
     out.wsum = ws2;
@@ -469,7 +497,7 @@ __global__ __launch_bounds__(64) void raster_fwd_kernel(RasterParams p) {
             }
         }
         stage_texels(tex, p, prim);
-        const RTex tx = texel_fwd_raster(o, d, tn, c, nrm, r, !tc.oob && sh.hit, tex, p.K);
+        const RTex tx = texel_fwd_raster<false>(o, d, tn, c, nrm, r, !tc.oob && sh.hit, tex, p.K);
         tf = tx.dp >= 0.0f ? fmin_(tx.t_surf, tf) : tf;
         tn = tx.dp < 0.0f ? fmax_(tx.t_surf, tn) : tn;
         const float dt = tf - tn;
@@ -548,19 +576,24 @@ __device__ __forceinline__ void texel_bwd_raster(f3 o, f3 d, float tn_in, f3 c, 
     const f3 q = ray_at(o, tq, d);
     f3 dq = mk(0, 0, 0);
     const float gdp_in = dot(g_rgb, fw.rgb);
+    // The forward's weights (no second exp), and reciprocals in place of the
+    // per-texel divisions; the 1e-3 skip keeps its exact division, so the set
+    // of contributing texels is the forward's.
+    const float k2 = (2.0f * kTemp) * (1.0f / r2);  // d w / d s = k2 w (q - s)
+    const float inv_ws = 1.0f / fw.wsum;
 #pragma unroll
     for (int i = 0; i < (KT > 0 ? KT : 8); ++i) {
         if (KT == 0 && i >= K) break;
         const f3 dv = q - mk(tex[3 * i], tex[3 * i + 1], tex[3 * i + 2]);
-        const float w = pf_expf(-kTemp * (dot(dv, dv) / r2));
+        const float w = fw.w1[i];
         if (w / fw.wsum < 1e-3f) continue;
-        const f3 dwds = divs(((2.0f * kTemp) * w) * dv, r2);
-        const f3 grgb = divs(w * g_rgb, fw.wsum);
+        const f3 dwds = (k2 * w) * dv;
+        const f3 grgb = (w * inv_ws) * g_rgb;
         red[3 * K + 3 * i] += grgb.x;
         red[3 * K + 3 * i + 1] += grgb.y;
         red[3 * K + 3 * i + 2] += grgb.z;
         const float gdp = dot(g_rgb, mk(tex[3 * K + 3 * i], tex[3 * K + 3 * i + 1], tex[3 * K + 3 * i + 2]));
-        const f3 gs = divs(dwds * (gdp - gdp_in), fw.wsum);
+        const f3 gs = dwds * ((gdp - gdp_in) * inv_ws);
         red[3 * i] += gs.x;
         red[3 * i + 1] += gs.y;
         red[3 * i + 2] += gs.z;
@@ -582,16 +615,17 @@ __device__ __forceinline__ void texel_bwd_raster(f3 o, f3 d, float tn_in, f3 c, 
     const f3 q0 = ray_at(o, tq0, d);
     f3 dq0 = mk(0, 0, 0);
     const float hdp_in = dh * fw.height;
+    const float inv_ws0 = 1.0f / fw.wsum0;
 #pragma unroll
     for (int i = 0; i < (KT > 0 ? KT : 8); ++i) {
         if (KT == 0 && i >= K) break;
         const f3 dv = q0 - mk(tex[3 * i], tex[3 * i + 1], tex[3 * i + 2]);
-        const float w = pf_expf(-kTemp * (dot(dv, dv) / r2));
+        const float w = fw.w0[i];
         if (w / fw.wsum0 < 1e-3f) continue;
-        const f3 dwds = divs(((2.0f * kTemp) * w) * dv, r2);
-        red[6 * K + i] += (w * dh) / fw.wsum0;
+        const f3 dwds = (k2 * w) * dv;
+        red[6 * K + i] += (w * dh) * inv_ws0;
         const float hdp = dh * tex[6 * K + i];
-        const f3 gs = divs(dwds * (hdp - hdp_in), fw.wsum0);
+        const f3 gs = dwds * ((hdp - hdp_in) * inv_ws0);
         red[3 * i] += gs.x;
         red[3 * i + 1] += gs.y;
         red[3 * i + 2] += gs.z;
@@ -680,7 +714,7 @@ __global__ __launch_bounds__(64) void raster_bwd_kernel(RasterParams p) {
         const float tn0 = tn;
         const int tn0_id = tn_id, tf0_id = tf_id;
         stage_texels(tex, p, prim);
-        const RTex tx = texel_fwd_raster(o, d, tn, c, nrm, r, !tc.oob && sh.hit, tex, K);
+        const RTex tx = texel_fwd_raster<true>(o, d, tn, c, nrm, r, !tc.oob && sh.hit, tex, K);
         tf_id = (tx.dp >= 0.0f && tx.t_surf < tf) ? n_adj : tf_id;
         tf = tx.dp >= 0.0f ? fmin_(tx.t_surf, tf) : tf;
         tn_id = (tx.dp < 0.0f && tx.t_surf > tn) ? n_adj : tn_id;
