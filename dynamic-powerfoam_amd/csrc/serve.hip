@@ -8,9 +8,10 @@
 // table.  Each wavefront takes one 8x8 tile; the request owning a tile is
 // found by binary search over the table's step_tile0 prefix.  Forward
 // requests write colour and log transmittance; forward+backward requests
-// also replay the walk for the gradient of sum_pixels |C - target|^2
-// (checkpoint: the colour just computed, kept in registers), through the
-// wave's LDS gradient cache.  The two
+// also compute the gradient of sum_pixels |C - target|^2 (checkpoint: the
+// colour just computed, kept in registers) through the wave's LDS gradient
+// cache, from the segments their forward recorded (or, without a record
+// workspace or past its cap, by replaying the walk).  The two
 // kinds run in two launches of one step, so forward tiles are not held to
 // the occupancy of the backward's register footprint.
 #include "pf_device.h"
@@ -48,19 +49,27 @@ __device__ __forceinline__ int serve_ray(const SceneDev<float> &s, const RGB &co
 constexpr int kServeSlots = 38;
 using ServeCache = CellCache<kServeSlots, 8>;
 
-// One ray of a forward+backward request: the forward, then the replayed
-// backward of sum |C - target|^2 (the checkpoint is the colour just
-// computed).  A wave serves one tile of one request, so its gradients go
-// through the wave's LDS cache (rows) into that request's arrays; every
-// lane takes part (valid == false: a pixel outside the image).
+// One ray of a forward+backward request: the forward, then the backward of
+// sum |C - target|^2 (the checkpoint is the colour just computed).  A wave
+// serves one tile of one request, so its gradients go through the wave's
+// LDS cache (rows) into that request's arrays; every lane takes part
+// (valid == false: a pixel outside the image).  With a record (rec.cap >
+// 0) the forward records the ray's contributing segments in its slot and
+// the backward reads them back (this lane's own writes); rays past the cap
+// are replayed.
 template <typename RGB>
 __device__ __forceinline__ int serve_ray_bwd(const SceneDev<float> &s, const RGB &colors,
                                              const pf_serve_request &rq, const Cam &cam, f3 d, int64_t pix,
-                                             float thr, int max_steps, float *rows, bool valid) {
+                                             float thr, int max_steps, float *rows, bool valid, SegRecord rec) {
     RayFwd r{};
     f3 gc = mk(0, 0, 0);
     if (valid) {
-        r = walk_ray_fwd<float, PackedF16, 8>(s, colors, cam.eye, d, cam.start, thr, 0.5f, max_steps, nullptr, 0);
+        if (rec.cap > 0)
+            r = walk_ray_fwd<float, PackedF16, 8, RGB, true>(s, colors, cam.eye, d, cam.start, thr, 0.5f, max_steps,
+                                                             nullptr, 0, rec);
+        else
+            r = walk_ray_fwd<float, PackedF16, 8>(s, colors, cam.eye, d, cam.start, thr, 0.5f, max_steps, nullptr,
+                                                  0);
         rq.color[3 * pix] = r.rgb.x;
         rq.color[3 * pix + 1] = r.rgb.y;
         rq.color[3 * pix + 2] = r.rgb.z;
@@ -74,18 +83,30 @@ __device__ __forceinline__ int serve_ray_bwd(const SceneDev<float> &s, const RGB
     cache.init(rows, G);
     uint64_t bytes = 0;
     const bool live = valid && (gc.x != 0.0f || gc.y != 0.0f || gc.z != 0.0f);
-    return r.status | walk_ray_bwd<float, PackedF16, 8, RGB, ServeCache>(s, colors, cam.eye, d, cam.start, thr,
-                                                                         max_steps, r.rgb, gc, 0.0f, G, false,
-                                                                         bytes, live, cache);
+    int st = r.status;
+    bool replay = live;
+    if (rec.cap > 0) {
+        const bool held = r.segs <= rec.cap;
+        st |= walk_rec_bwd<float, PackedF16, 8, RGB, ServeCache>(s, colors, cam.eye, d, rec.rec, rec.slot, rec.S,
+                                                                 r.segs, r.rgb, gc, 0.0f, G, false, bytes,
+                                                                 live && held, cache);
+        replay = live && !held;
+    }
+    return st | walk_ray_bwd<float, PackedF16, 8, RGB, ServeCache>(s, colors, cam.eye, d, cam.start, thr, max_steps,
+                                                                   r.rgb, gc, 0.0f, G, false, bytes, replay, cache);
 }
 
 // Served scenes use f32 attributes, K = 8 and packed walk faces
 // (PF_F16_PACKED).  The table lists forward entries first: tiles [0, desc[2]) are forward
 // only and [desc[2], desc[1]) forward+backward.  BWD selects the range, so
 // forward-only tiles run in a kernel without the backward's registers.
+// rec_segs / rec_cap: the forward+backward tiles' record workspace (one
+// slot per lane of the launch, 64 * 4 * gridDim.x slots of rec_cap
+// segments), or rec_cap == 0 to replay.
 template <int DEG, bool BWD, bool SH>
-__global__ __launch_bounds__(256) void serve_kernel(const pf_serve_scene *scenes, const pf_serve_request *reqs,
-                                                    const int64_t *desc, float thr, int max_steps, int *status) {
+__device__ __forceinline__ void serve_tile(const pf_serve_scene *scenes, const pf_serve_request *reqs,
+                                           const int64_t *desc, float thr, int max_steps, int *status,
+                                           int4 *rec_segs, int rec_cap) {
     const int nreq = (int)desc[0];
     const int64_t begin = BWD ? desc[2] : 0, end = BWD ? desc[1] : desc[2];
     const int64_t g = begin + (int64_t)xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
@@ -112,15 +133,40 @@ __global__ __launch_bounds__(256) void serve_kernel(const pf_serve_scene *scenes
     if constexpr (BWD) {
         __shared__ float rows[4][kServeSlots * ServeCache::kPitch];
         float *wave_rows = rows[threadIdx.x >> 6];
+        const SegRecord rec{rec_segs, (g - begin) * 64 + lane, (int64_t)gridDim.x * 4 * 64, rec_cap};
         if (sh_in_walk)
-            st = serve_ray_bwd(s, SHRGB<DEG>{sc.sh, cam.eye}, rq, cam, d, pix, thr, max_steps, wave_rows, valid);
+            st = serve_ray_bwd(s, SHRGB<DEG>{sc.sh, cam.eye}, rq, cam, d, pix, thr, max_steps, wave_rows, valid, rec);
         else
-            st = serve_ray_bwd(s, MemRGB<float>{rgb}, rq, cam, d, pix, thr, max_steps, wave_rows, valid);
+            st = serve_ray_bwd(s, MemRGB<float>{rgb}, rq, cam, d, pix, thr, max_steps, wave_rows, valid, rec);
     } else {
         if (sh_in_walk) st = serve_ray(s, SHRGB<DEG>{sc.sh, cam.eye}, rq, cam, d, pix, thr, max_steps);
         else st = serve_ray(s, MemRGB<float>{rgb}, rq, cam, d, pix, thr, max_steps);
     }
     if (st && status) atomicOr(status, st);
+}
+
+template <int DEG, bool SH>
+__global__ __launch_bounds__(256) void serve_fwd_kernel(const pf_serve_scene *scenes, const pf_serve_request *reqs,
+                                                        const int64_t *desc, float thr, int max_steps, int *status) {
+    serve_tile<DEG, false, SH>(scenes, reqs, desc, thr, max_steps, status, nullptr, 0);
+}
+
+// Colours from memory: four waves per SIMD, as the walk backward (the LDS
+// gradient cache allows four blocks per CU; the register budget is capped
+// to match).  SH evaluated in the walk needs more registers than that cap
+// and runs uncapped.
+template <int DEG>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void serve_bwd_kernel(
+    const pf_serve_scene *scenes, const pf_serve_request *reqs, const int64_t *desc, float thr, int max_steps,
+    int *status, int4 *rec_segs, int rec_cap) {
+    serve_tile<DEG, true, false>(scenes, reqs, desc, thr, max_steps, status, rec_segs, rec_cap);
+}
+template <int DEG>
+__global__ __launch_bounds__(256) void serve_bwd_sh_kernel(const pf_serve_scene *scenes,
+                                                           const pf_serve_request *reqs, const int64_t *desc,
+                                                           float thr, int max_steps, int *status, int4 *rec_segs,
+                                                           int rec_cap) {
+    serve_tile<DEG, true, true>(scenes, reqs, desc, thr, max_steps, status, rec_segs, rec_cap);
 }
 
 }  // namespace pf
@@ -129,24 +175,36 @@ using namespace pf;
 
 template <bool SH>
 static void launch_step(const pf_serve_scene *scenes, const pf_serve_request *requests, const int64_t *desc,
-                        unsigned blocks, float thr, int32_t max_steps, int32_t *status, hipStream_t st) {
-    hipLaunchKernelGGL((serve_kernel<3, false, SH>), dim3(blocks), dim3(256), 0, st, scenes, requests, desc, thr,
+                        unsigned blocks, float thr, int32_t max_steps, int32_t *status, int4 *rec_segs, int rec_cap,
+                        hipStream_t st) {
+    hipLaunchKernelGGL((serve_fwd_kernel<3, SH>), dim3(blocks), dim3(256), 0, st, scenes, requests, desc, thr,
                        max_steps, status);
-    hipLaunchKernelGGL((serve_kernel<3, true, SH>), dim3(blocks), dim3(256), 0, st, scenes, requests, desc, thr,
-                       max_steps, status);
+    if constexpr (SH)
+        hipLaunchKernelGGL((serve_bwd_sh_kernel<3>), dim3(blocks), dim3(256), 0, st, scenes, requests, desc, thr,
+                           max_steps, status, rec_segs, rec_cap);
+    else
+        hipLaunchKernelGGL((serve_bwd_kernel<3>), dim3(blocks), dim3(256), 0, st, scenes, requests, desc, thr,
+                           max_steps, status, rec_segs, rec_cap);
+}
+
+extern "C" int64_t pf_serve_record_slots(int64_t tile_capacity) {
+    return tile_capacity > 0 ? (tile_capacity + 3) / 4 * 4 * 64 : 0;
 }
 
 extern "C" int pf_serve_step(const pf_serve_scene *scenes, const pf_serve_request *requests, const int64_t *desc,
                              int64_t tile_capacity, float transmittance_threshold, int32_t max_steps, int32_t flags,
-                             int32_t *status, void *stream) {
+                             int32_t *status, const pf_walk_record *record, void *stream) {
     if (!scenes || !requests || !desc || tile_capacity <= 0 || max_steps <= 0) return PF_ERR_ARG;
     if (flags & ~PF_SERVE_SH_IN_WALK) return PF_ERR_ARG;
+    if (record && (!record->segments || record->cap <= 0 || record->cap % kRecChunk)) return PF_ERR_ARG;
     const unsigned blocks = (unsigned)((tile_capacity + 3) / 4);
+    int4 *segs = record ? reinterpret_cast<int4 *>(record->segments) : nullptr;
+    const int cap = record ? record->cap : 0;
     if (flags & PF_SERVE_SH_IN_WALK)
-        launch_step<true>(scenes, requests, desc, blocks, transmittance_threshold, max_steps, status,
+        launch_step<true>(scenes, requests, desc, blocks, transmittance_threshold, max_steps, status, segs, cap,
                           (hipStream_t)stream);
     else
-        launch_step<false>(scenes, requests, desc, blocks, transmittance_threshold, max_steps, status,
+        launch_step<false>(scenes, requests, desc, blocks, transmittance_threshold, max_steps, status, segs, cap,
                            (hipStream_t)stream);
     return hipGetLastError() == hipSuccess ? PF_OK : PF_ERR_HIP;
 }

@@ -204,7 +204,12 @@ _SIGS = {
         c_int32,
         [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_float, c_int32, c_void_p, c_void_p, c_void_p, c_void_p],
     ),
-    "pf_serve_step": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_float, c_int32, c_int32, c_void_p, c_void_p]),
+    "pf_serve_record_slots": (c_int64, [c_int64]),
+    "pf_serve_step": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_int64, c_float, c_int32, c_int32, c_void_p, ctypes.POINTER(PfWalkRecord),
+         c_void_p],
+    ),
     "pf_sh_forward": (
         c_int32,
         [c_void_p, c_int32, c_void_p, c_int64, c_int32, c_void_p, c_int32, c_void_p, c_int32, c_int64, c_void_p],

@@ -132,7 +132,7 @@ class ResidentScene:
 class RenderServer:
     def __init__(self, device="cuda", memory_budget=64 << 30, tile_capacity=16384, quantum=1024, use_graph=True,
                  max_wait=2.0, transmittance_threshold=1e-3, max_steps=1 << 16, colour="prepass", bwd_weight=1,
-                 oldest_share=0.0, streams=4):
+                 oldest_share=0.0, streams=4, record_cap=32):
         if colour not in ("prepass", "walk"):
             raise ValueError("colour must be 'prepass' or 'walk'")
         self.colour = colour
@@ -172,6 +172,14 @@ class RenderServer:
         # the next step's start
         self._streams = [torch.cuda.Stream(device=device) for _ in range(self.nbuf)]
         self._flags = _lib.PF_SERVE_SH_IN_WALK if colour == "walk" else 0
+        # forward+backward tiles record their contributing segments (up to
+        # record_cap per ray, a multiple of 8; 0 replays the walk) in one
+        # workspace per table buffer, allocated with the first such request:
+        # 16 * record_cap * 64 bytes per tile of capacity and buffer, fixed
+        # server memory outside memory_budget (2.1 GB at the defaults)
+        self.record_cap = -(-int(record_cap) // 8) * 8 if record_cap > 0 else 0
+        self._records = None
+        self.record_bytes = 0
 
     # ------------------------------------------------------------ scenes --
     def add_scene(self, name, scene: ResidentScene):
@@ -200,7 +208,23 @@ class RenderServer:
             b += self.scenes[r.scene].grad_bytes() + r.width * r.height * 12
         return b
 
+    def _ensure_records(self):
+        if self._records is not None or self.record_cap <= 0:
+            return
+        slots = _lib.load().pf_serve_record_slots(self.capacity)
+        self._records = []
+        for _ in range(self.nbuf):
+            segs = torch.empty(self.record_cap * slots, 4, dtype=torch.int32, device=self.device)
+            rec = _lib.PfWalkRecord()
+            rec.segments = segs.data_ptr()
+            rec.cap = self.record_cap
+            self._records.append((segs, rec))
+        self.record_bytes = 16 * self.record_cap * slots * self.nbuf
+        self._graphs = None  # the launches now carry the workspaces
+
     def submit(self, r: RenderRequest):
+        if r.mode == "forward_backward":
+            self._ensure_records()
         r._bytes = self._request_bytes(r)
         if r._bytes > self.budget - sum(s.nbytes() for s in self.scenes.values()):
             r.rejected = True
@@ -310,12 +334,16 @@ class RenderServer:
         lib = _lib.load()
         host, dev = self._host_tab[buf], self._dev_tab[buf]
         dev.copy_(host, non_blocking=True)
+        rec = ctypes.byref(self._records[buf][1]) if self._records is not None else None
         rc = lib.pf_serve_step(_lib.ptr(self._scene_tab), dev.data_ptr(),
                                dev.data_ptr() + self.max_reqs * self._req_bytes, self.capacity, self.thr,
-                               self.max_steps, self._flags, _lib.ptr(self.status), _lib.stream_ptr())
+                               self.max_steps, self._flags, _lib.ptr(self.status), rec, _lib.stream_ptr())
         _lib.check(rc, "pf_serve_step")
 
     def _build_graphs(self):
+        # steps in flight may still read their tables, which the warm-up
+        # below rewrites, and the old graphs: let them finish first
+        torch.cuda.synchronize()
         self._graphs = []
         for buf in range(self.nbuf):
             s = self._streams[buf]

@@ -436,11 +436,18 @@ typedef struct pf_serve_request {
                                      in the walk (otherwise every entry reads
                                      colours from memory: its own texel_rgb
                                      or the scene's)                        */
+/* record: NULL (forward+backward entries replay the walk), or a workspace
+ * whose segments hold cap * pf_serve_record_slots(tile_capacity) records
+ * (counts unused): each forward+backward ray records its contributing
+ * segments there and its backward reads them back; rays with more than cap
+ * are replayed.  Steps that may run at the same time need separate
+ * workspaces.                                                             */
+int64_t pf_serve_record_slots(int64_t tile_capacity);
 int pf_serve_step(const pf_serve_scene *scenes,
                   const pf_serve_request *requests, const int64_t *desc,
                   int64_t tile_capacity, float transmittance_threshold,
                   int32_t max_steps, int32_t flags, int32_t *status,
-                  void *stream);
+                  const pf_walk_record *record, void *stream);
 
 /* ---- power-diagram adjacency ----------------------------------------- */
 
