@@ -51,6 +51,9 @@ def main():
     ap.add_argument("--adjacency", choices=("gpu", "qhull"), default="gpu",
                     help="power-diagram adjacency from the GPU builder or the CPU Qhull lift (same edges)")
     ap.add_argument("--serve-quantum", type=int, default=1024, help="tiles per request per step before leftovers")
+    ap.add_argument("--serve-streams", type=int, default=4, help="server streams (table buffers)")
+    ap.add_argument("--serve-oldest-share", type=float, default=0.0,
+                    help="share of a step's capacity the oldest active request may take")
     ap.add_argument("--record-cap", type=int, default=64,
                     help="contributing segments per ray the forward records for the backward (0 = replay)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -381,7 +384,7 @@ def serve_trace(scenes, make, args, label):
     p_wall = plain.run(p_trace)
     p_stats = latency_stats(p_trace)
     srv = RenderServer("cuda", memory_budget=64 << 30, tile_capacity=args.serve_capacity, quantum=args.serve_quantum,
-                       oldest_share=getattr(args, "serve_oldest_share", 0.0))
+                       oldest_share=args.serve_oldest_share, streams=args.serve_streams)
     for name, scene in scenes.items():
         srv.add_scene(name, scene)
     srv.on_finish = consume

@@ -109,8 +109,13 @@ __device__ __forceinline__ void serve_tile(const pf_serve_scene *scenes, const p
                                            int4 *rec_segs, int rec_cap) {
     const int nreq = (int)desc[0];
     const int64_t begin = BWD ? desc[2] : 0, end = BWD ? desc[1] : desc[2];
-    const int64_t g = begin + (int64_t)xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
-    if (g >= end || nreq <= 0) return;
+    // The grid covers the capacity; the XCD-aware numbering is taken over
+    // this step's blocks only, so that a small step spreads over every XCD
+    // (over the whole grid it would put a short prefix of tiles on one).
+    const int nact = (int)((end - begin + 3) / 4);
+    if ((int)blockIdx.x >= nact || nreq <= 0) return;
+    const int64_t g = begin + (int64_t)xcd_block(blockIdx.x, nact) * 4 + (threadIdx.x >> 6);
+    if (g >= end) return;
     const int lane = threadIdx.x & 63;
     const int ri = __builtin_amdgcn_readfirstlane(find_request(reqs, nreq, g));
     const pf_serve_request &rq = reqs[ri];
