@@ -51,7 +51,7 @@ def main():
     ap.add_argument("--adjacency", choices=("gpu", "qhull"), default="gpu",
                     help="power-diagram adjacency from the GPU builder or the CPU Qhull lift (same edges)")
     ap.add_argument("--serve-quantum", type=int, default=1024, help="tiles per request per step before leftovers")
-    ap.add_argument("--serve-streams", type=int, default=4, help="server streams (table buffers)")
+    ap.add_argument("--serve-streams", type=int, default=3, help="server streams (table buffers)")
     ap.add_argument("--serve-oldest-share", type=float, default=0.0,
                     help="share of a step's capacity the oldest active request may take")
     ap.add_argument("--record-cap", type=int, default=64,

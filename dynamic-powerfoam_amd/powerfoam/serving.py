@@ -10,12 +10,14 @@ oldest), so requests that arrive while others are on the GPU join at the
 next step and their tiles run in the same launch.  The request table lives
 in pinned host memory; each step copies it to the device and launches the
 kernels, and both are captured once in a HIP graph per table buffer.  The
-buffers rotate (``streams``, default 4), so the host fills one while the
+buffers rotate (``streams``, default 3), so the host fills one while the
 GPU runs the others, and each has its own default-priority stream, so one
-step's last tiles overlap the next step's first.  With two streams, runs
-came in two modes: in some the two streams shared a hardware queue, the
-steps stopped overlapping and the p99 of a mixed trace doubled; with four,
-consecutive steps land on different queues.
+step's last tiles overlap the next step's first.  A process gets four
+hardware queues by default (GPU_MAX_HW_QUEUES), one of which serves torch's
+current stream (admission work): with three server streams every stream
+has a queue of its own.  With two, some runs had both on one queue and the
+mixed trace's p99 doubled; with four, two streams shared a queue in some
+runs (multi-scene p99 67-75 ms against 47-54 ms with three).
 
 Colour: with ``colour="prepass"`` (default) each admitted request of an
 SH-coloured scene gets its own texel colours from one SH pre-pass for its
@@ -132,7 +134,7 @@ class ResidentScene:
 class RenderServer:
     def __init__(self, device="cuda", memory_budget=64 << 30, tile_capacity=16384, quantum=1024, use_graph=True,
                  max_wait=2.0, transmittance_threshold=1e-3, max_steps=1 << 16, colour="prepass", bwd_weight=1,
-                 oldest_share=0.0, streams=4, record_cap=32):
+                 oldest_share=0.0, streams=3, record_cap=32):
         if colour not in ("prepass", "walk"):
             raise ValueError("colour must be 'prepass' or 'walk'")
         self.colour = colour
