@@ -91,6 +91,46 @@ def test_served_equals_plain_path(served):
             assert np.linalg.norm(x - y) <= 1e-4 * np.linalg.norm(y)
 
 
+@pytest.mark.parametrize("record_cap", [0, 8])
+def test_served_backward_with_and_without_record(served, record_cap):
+    """Forward+backward tiles differentiated by replay (record_cap 0) or from
+    a small record (8: rays with more contributing segments are replayed in
+    the same wave) give the plain path's gradients."""
+    reqs = [r for r in _requests(11, 9) if r.mode == "forward_backward"]
+    srv = RenderServer(DEV, tile_capacity=256, quantum=64, record_cap=record_cap)
+    srv.add_scene("s", served)
+    _drain(srv, reqs)
+    assert int(srv.status.item()) == 0
+    plain = PlainRenderer({"s": served})
+    for r in [r for r in _requests(11, 9) if r.mode == "forward_backward"]:
+        plain.render(r)
+        q = next(x for x in reqs if x.rid == r.rid)
+        assert torch.equal(q.color, r.color)
+        for k in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
+            x, y = q.grads[k].cpu().numpy(), r.grads[k].cpu().numpy()
+            assert np.linalg.norm(x - y) <= 1e-4 * max(np.linalg.norm(y), 1e-30), k
+    if record_cap:
+        # the record held some rays and not others: their counts, from a
+        # recording forward of the same requests
+        from powerfoam import _lib, shading
+        from powerfoam.camera import cameras_to_device
+        from powerfoam.serving import _SceneView
+
+        over = held = 0
+        for r in reqs:
+            cam = cameras_to_device([r.camera], None, DEV)
+            rt.start_cells_device(served.points, served.radii, cam)
+            rgb = shading.sh_forward(served.sh, 3, served.walk.texel_sites, cam)
+            c = _lib.PfScene.from_buffer_copy(served.walk.c)
+            c.texel_rgb = rgb.data_ptr()
+            f = rt.launch_walk_forward(_SceneView(c, served.num_cells, served.K, rgb), cam, r.width, r.height,
+                                       record_cap=512)
+            counts = rt.record_counts(f["record"], 1, r.width, r.height)
+            over += int((counts > record_cap).sum())
+            held += int(((counts > 0) & (counts <= record_cap)).sum())
+        assert over > 0 and held > 0
+
+
 def test_graph_replays_match_eager_repeatedly(served):
     outs = []
     for use_graph in (False, True, True):
