@@ -33,8 +33,10 @@ __global__ __launch_bounds__(256) void sh_fwd_kernel(const float *coeffs, const 
 }
 
 // One thread per cell: sums over its K texels and all B cameras, writes
-// (accumulates) dL/dcoeffs once, no atomics.  The clamp at 0 passes the
-// gradient only where the unclamped value is positive.
+// (accumulates) dL/dcoeffs once, no atomics.  The cell's coefficients are
+// read once into registers (read per texel, they were fetched from L2 and
+// HBM K times over).  The clamp at 0 passes the gradient only where the
+// unclamped value is positive.
 template <int DEG>
 __global__ __launch_bounds__(256) void sh_bwd_kernel(const float *coeffs, const float *sites, int64_t N, int K,
                                                      const float *cams, int B, const float *grad_rgb,
@@ -42,7 +44,9 @@ __global__ __launch_bounds__(256) void sh_bwd_kernel(const float *coeffs, const 
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     constexpr int NC = (DEG + 1) * (DEG + 1);
-    const float *k = coeffs + i * NC * 3;
+    float k[NC * 3];
+#pragma unroll
+    for (int l = 0; l < NC * 3; ++l) k[l] = coeffs[i * NC * 3 + l];
     float acc[NC * 3];
 #pragma unroll
     for (int l = 0; l < NC * 3; ++l) acc[l] = 0.0f;
