@@ -538,7 +538,14 @@ def cpu_baseline(sc, adj, off, faces, texel_rgb, cams_dev, fwd, gc, seconds):
     n = int(min(W * H, max(probe, probe * seconds / max(tp, 1e-6))))
     log(f"cpu baseline: {nth} threads, probe {probe} rays in {tp:.2f}s; sample {n} rays")
     tt = run(n)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(n / tt / 1e6, 5), "unit": "Mrays/s", "cores": nth, "kind": "port",
+            "host": {"nproc": os.cpu_count(), "cpu_model": model},
             "sample": f"walk fwd+bwd (C++ restatement, OpenMP) on {n} evenly spaced rays of the same 1080p frame, "
                       f"{tt:.1f}s; SH pre-pass excluded"}
 
