@@ -54,7 +54,7 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=3, help="server streams (table buffers)")
     ap.add_argument("--serve-oldest-share", type=float, default=0.0,
                     help="share of a step's capacity the oldest active request may take")
-    ap.add_argument("--record-cap", type=int, default=64,
+    ap.add_argument("--record-cap", type=int, default=128,
                     help="contributing segments per ray the forward records for the backward (0 = replay)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (reference kernel for HBM byte counters)")

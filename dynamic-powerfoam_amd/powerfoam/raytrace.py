@@ -20,7 +20,7 @@ TILE_SIZE = TILE_WIDTH * TILE_WIDTH
 DEFAULT_MAX_STEPS = 1 << 16
 # contributing segments per ray the differentiable forward records for its
 # backward, and the most device memory one record may take
-DEFAULT_RECORD_CAP = 64
+DEFAULT_RECORD_CAP = 128
 DEFAULT_RECORD_BUDGET = 8 << 30
 
 
