@@ -649,8 +649,11 @@ __device__ __forceinline__ void texel_bwd_raster(f3 o, f3 d, float tn_in, f3 c, 
 constexpr int kRedCols = 8 + 7 * 8;  // K <= 8 on this path
 constexpr int kRedPitch = 33;
 
+// Three waves per SIMD: the uncapped kernel takes 206 VGPRs (two waves,
+// with LDS for three); capped at 168 it spills 9 and measured 10% faster
+// (1M-cell scene at 1920x1080: forward+backward 50.9 against 56.1 ms).
 template <typename F, int KT>
-__global__ __launch_bounds__(64) void raster_bwd_kernel(RasterParams p) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void raster_bwd_kernel(RasterParams p) {
     using FT = typename RFace<F>::T;
     __shared__ PrimChunk pc;
     __shared__ FT sface[64];
