@@ -15,7 +15,10 @@
   each other.
 
 Rays outside the sample get a zero incoming gradient, so both backwards
-replay only the sampled rays.
+replay only the sampled rays.  Each case also runs the recorded backward
+(the forward's record of contributing segments, read instead of a replay)
+and checks it against the oracle and the replay; in the 16-camera case at a
+cap that some rays exceed, so that the same launch replays those.
 """
 import numpy as np
 import pytest
@@ -33,6 +36,12 @@ DEV = "cuda"
 
 def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def rt_status_cycle():
+    from powerfoam import _lib
+
+    return _lib.PF_STATUS_CYCLE  # the frame holds a looping ray (stopped, as in the forward)
 
 
 def _setup(n, cams_fn, seed, f16=True):
@@ -84,10 +93,17 @@ def test_1m_cells_1080p_sampled_rays():
     mask[torch.from_numpy(rays).to(DEV)] = True
     gc = (2.0 * (g["color"] - target)) * mask.view(1, H, W, 1)
     gg = rt.launch_walk_backward(gs, cams_dev, W, H, g["color"], g["log_t"], gc, None)
+    # the benchmark step's form: the forward's record, read by the backward
+    gr = rt.launch_walk_forward(gs, cams_dev, W, H, want_depth=False, record_cap=128)
+    assert torch.equal(gr["color"], g["color"])
+    grec = rt.launch_walk_backward(gs, cams_dev, W, H, g["color"], g["log_t"], gc, None, record=gr["record"])
     torch.cuda.synchronize()
+    assert int(grec["status"].item()) & ~rt_status_cycle() == 0
     cb = orc.walk_backward(cs, words, g["color"].cpu().numpy(), gc.cpu().numpy())
     for k in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
         assert _rel(gg[k].cpu().numpy(), cb[k]) < 1e-3, k
+        assert _rel(grec[k].cpu().numpy(), cb[k]) < 1e-3, k
+        assert _rel(grec[k].cpu().numpy(), gg[k].cpu().numpy()) < 1e-5, k
 
 
 def test_262k_cells_16_cameras_finite_differences():
@@ -107,7 +123,15 @@ def test_262k_cells_16_cameras_finite_differences():
     gc = 2.0 * (fwd["color"] - torch.from_numpy(target).to(DEV)) * mask.view(B, H, W, 1)
     gl = -w_alpha * torch.exp(fwd["log_t"]) * mask.view(B, H, W)
     g = rt.launch_walk_backward(gs, cams_dev, W, H, fwd["color"], fwd["log_t"], gc, gl, exact_texel=True)
+    # the recorded backward at a cap some rays exceed (both halves of its launch)
+    fr = rt.launch_walk_forward(gs, cams_dev, W, H, record_cap=32)
+    gr = rt.launch_walk_backward(gs, cams_dev, W, H, fwd["color"], fwd["log_t"], gc, gl, exact_texel=True,
+                                 record=fr["record"])
     torch.cuda.synchronize()
+    assert bool((fr["record"]["counts"] > 32).any())
+    for k in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
+        assert _rel(gr[k].cpu().numpy(), g[k].cpu().numpy()) < 1e-5, k
+    del fr, gr
     g = {k: v.cpu().numpy() for k, v in g.items() if v is not None and k != "status"}
     # the oracle's backward on every parameter
     rgb0 = rgb[0].cpu().numpy()
