@@ -52,6 +52,8 @@ def main():
                     help="power-diagram adjacency from the GPU builder or the CPU Qhull lift (same edges)")
     ap.add_argument("--serve-quantum", type=int, default=1024, help="tiles per request per step before leftovers")
     ap.add_argument("--serve-streams", type=int, default=3, help="server streams (table buffers)")
+    ap.add_argument("--serve-record-cap", type=int, default=128,
+                    help="segments per ray the server records for forward+backward tiles (0 = replay)")
     ap.add_argument("--serve-oldest-share", type=float, default=0.0,
                     help="share of a step's capacity the oldest active request may take")
     ap.add_argument("--record-cap", type=int, default=128,
@@ -384,7 +386,8 @@ def serve_trace(scenes, make, args, label):
     p_wall = plain.run(p_trace)
     p_stats = latency_stats(p_trace)
     srv = RenderServer("cuda", memory_budget=64 << 30, tile_capacity=args.serve_capacity, quantum=args.serve_quantum,
-                       oldest_share=args.serve_oldest_share, streams=args.serve_streams)
+                       oldest_share=args.serve_oldest_share, streams=args.serve_streams,
+                       record_cap=args.serve_record_cap)
     for name, scene in scenes.items():
         srv.add_scene(name, scene)
     srv.on_finish = consume

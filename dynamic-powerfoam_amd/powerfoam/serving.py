@@ -134,7 +134,7 @@ class ResidentScene:
 class RenderServer:
     def __init__(self, device="cuda", memory_budget=64 << 30, tile_capacity=16384, quantum=1024, use_graph=True,
                  max_wait=2.0, transmittance_threshold=1e-3, max_steps=1 << 16, colour="prepass", bwd_weight=1,
-                 oldest_share=0.0, streams=3, record_cap=32):
+                 oldest_share=0.0, streams=3, record_cap=128):
         if colour not in ("prepass", "walk"):
             raise ValueError("colour must be 'prepass' or 'walk'")
         self.colour = colour
@@ -178,7 +178,7 @@ class RenderServer:
         # record_cap per ray, a multiple of 8; 0 replays the walk) in one
         # workspace per table buffer, allocated with the first such request:
         # 16 * record_cap * 64 bytes per tile of capacity and buffer, fixed
-        # server memory outside memory_budget (2.1 GB at the defaults)
+        # server memory outside memory_budget (6.4 GB at the defaults)
         self.record_cap = -(-int(record_cap) // 8) * 8 if record_cap > 0 else 0
         self._records = None
         self.record_bytes = 0
