@@ -182,3 +182,26 @@ def test_record_of_another_walk_is_reported_not_read(scene_textured):
     torch.cuda.synchronize()
     assert int(g["status"].item()) & _lib.PF_STATUS_BAD_RECORD
     assert all(bool(g[k].isfinite().all()) for k in KEYS)
+
+
+@pytest.mark.parametrize("attr,f16,packed", [(torch.float16, True, True), (torch.float32, False, False),
+                                             (torch.float16, False, False), (torch.float32, True, False),
+                                             (torch.float16, True, False)])
+def test_recorded_backward_dtypes(scene_textured, attr, f16, packed):
+    """Every attribute dtype x face layout: the recording forward equals the
+    plain one and the recorded backward equals the replay."""
+    sc = scene_textured
+    cams = [workloads.front_camera(40, 32, hfov_deg=40.0)]
+    words = _cams(sc, cams)
+    gs, _ = _gpu_scene(sc, attr, f16, packed)
+    cams_dev = torch.from_numpy(words).to(DEV)
+    plain = rt.launch_walk_forward(gs, cams_dev, 40, 32, want_sequences=True)
+    fwd = rt.launch_walk_forward(gs, cams_dev, 40, 32, want_sequences=True, record_cap=16)
+    for key in ("color", "log_t", "depth", "visits", "seq_hash"):
+        assert torch.equal(plain[key], fwd[key]), key
+    gc = 2.0 * (fwd["color"] - 0.5)
+    rec = rt.launch_walk_backward(gs, cams_dev, 40, 32, fwd["color"], fwd["log_t"], gc, None, record=fwd["record"])
+    rep = rt.launch_walk_backward(gs, cams_dev, 40, 32, fwd["color"], fwd["log_t"], gc, None)
+    torch.cuda.synchronize()
+    for key in KEYS:
+        assert _rel(rec[key].cpu().numpy(), rep[key].cpu().numpy()) < 1e-5, key
