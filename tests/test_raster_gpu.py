@@ -55,6 +55,21 @@ def test_cech_textured_and_isolated_spheres(scene_textured):
     assert np.array_equal(off.cpu().numpy(), wo) and np.array_equal(adj.cpu().numpy(), wa)
 
 
+def test_cech_60k_cells_equals_brute_force():
+    """A clustered 60,000-cell scene (radii tripled, so rows are not empty):
+    the grid and row offsets go through the multi-chunk scan (15 chunks of
+    rows, more of grid cells)."""
+    sc = workloads.gmm_scene(60_000, seed=3)
+    rad = (3.0 * sc["radii"]).astype(np.float32)
+    tree = AABBTree(DEV)
+    tree.update(t(sc["points"]), t(rad))
+    adj, off = tree.build_cech_complex()
+    wa, wo = orc.cech(sc["points"], rad)
+    assert wa.size > 60_000
+    assert np.array_equal(off.cpu().numpy(), wo)
+    assert np.array_equal(adj.cpu().numpy(), wa)
+
+
 def test_raster_faces_bit_equal(scene4k, cech4k):
     sph = np.concatenate([scene4k["points"], scene4k["radii"][:, None]], 1).astype(np.float32)
     adj, off = cech4k
