@@ -154,15 +154,6 @@ def test_autograd_record_and_replay_agree(scene_textured):
         assert _rel(a, b) < 1e-5
 
 
-def test_record_cap_fits_budget():
-    import types
-
-    tracer = rt.RayTracer(types.SimpleNamespace(walk_record_cap=64, walk_record_budget=1 << 30), DEV)
-    cap = tracer.record_cap_for(1, 1920, 1080)
-    assert 8 <= cap < 64 and cap % 8 == 0 and rt.record_bytes(1, 1920, 1080, cap) <= 1 << 30
-    assert tracer.record_cap_for(64, 1920, 1080) == 0
-
-
 def test_record_of_another_walk_is_reported_not_read(scene_textured):
     """A recorded cell outside the scene is caught by the backward's bounds
     check: PF_STATUS_BAD_RECORD, the ray's remaining segments skipped."""
