@@ -956,6 +956,14 @@ extern "C" int pf_raster_faces(const float *spheres, int64_t num_cells, const in
     return hip_ok();
 }
 
+// At most 2^24 8x8 tiles (1.07 G pixels) per image: the per-tile sort runs
+// one 256-thread workgroup per tile, under 2^32 work-items per dispatch.
+constexpr int64_t kMaxRasterTiles = int64_t(1) << 24;
+
+static bool raster_grid_fits(int32_t W, int32_t H) {
+    return (int64_t)((H + 7) / 8) * ((W + 7) / 8) <= kMaxRasterTiles;
+}
+
 extern "C" int pf_raster_bin_count(const float *spheres, int64_t num_cells, const pf_camera *cam,
                                    int32_t *tile_counts, void *stream) {
     if (!spheres || num_cells <= 0 || !cam || !tile_counts) return PF_ERR_ARG;
@@ -985,6 +993,7 @@ extern "C" int pf_raster_bin_write(const float *spheres, int64_t num_cells, cons
                                    uint64_t *scratch, int32_t *tile_prims, uint8_t *visible_mask,
                                    int64_t num_isect, void *stream) {
     if (!spheres || num_cells <= 0 || !cam || !tile_offsets || !write_counter || num_tiles <= 0) return PF_ERR_ARG;
+    if (num_tiles > kMaxRasterTiles) return PF_ERR_ARG;
     if (num_isect > 0 && (!keys || !scratch || !tile_prims)) return PF_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(bin_write_kernel, dim3((unsigned)((num_cells + 255) / 256)), dim3(256), 0, st,
@@ -1027,6 +1036,7 @@ extern "C" int pf_raster_forward(const pf_scene *scene, const pf_camera *cam, in
                                  const float *ray_gt, int32_t return_point_err, int32_t surface_objective,
                                  const pf_raster_out *out, void *stream) {
     if (!scene || !cam || !tile_offsets || !tile_early_stop || !out || width <= 0 || height <= 0) return PF_ERR_ARG;
+    if (!raster_grid_fits(width, height)) return PF_ERR_ARG;
     if (scene->attr_dtype != PF_F32) return PF_ERR_DTYPE;
     if (scene->num_texel_sites < 1 || scene->num_texel_sites > 16) return PF_ERR_ARG;
     if (!out->color || !out->log_t || !out->normal_distance || !out->normal || !out->contrib) return PF_ERR_ARG;
@@ -1066,6 +1076,7 @@ extern "C" int pf_raster_backward(const pf_scene *scene, const pf_camera *cam, i
                                   const pf_grads *grads, void *stream) {
     if (!scene || !cam || !tile_offsets || !tile_early_stop || !log_t || !gin || !grads || width <= 0 || height <= 0)
         return PF_ERR_ARG;
+    if (!raster_grid_fits(width, height)) return PF_ERR_ARG;
     if (scene->attr_dtype != PF_F32) return PF_ERR_DTYPE;
     if (scene->num_texel_sites < 1 || scene->num_texel_sites > 8) return PF_ERR_ARG;
     if (!gin->color) return PF_ERR_ARG;
@@ -1103,6 +1114,7 @@ extern "C" int pf_raster_benchmark(const pf_scene *scene, const pf_camera *cam, 
                                    const int64_t *tile_offsets, const int32_t *tile_prims,
                                    float transmittance_threshold, float *color, void *stream) {
     if (!scene || !cam || !tile_offsets || !color || width <= 0 || height <= 0) return PF_ERR_ARG;
+    if (!raster_grid_fits(width, height)) return PF_ERR_ARG;
     if (scene->face_dtype != PF_F16) return PF_ERR_DTYPE;
     if (scene->num_texel_sites < 1 || scene->num_texel_sites > 16) return PF_ERR_ARG;
     RasterParams p{};
@@ -1121,6 +1133,7 @@ extern "C" int pf_raster_visualize(const pf_scene *scene, const pf_camera *cam, 
                                    const int64_t *tile_offsets, const int32_t *tile_prims,
                                    const pf_vis_options *options, const pf_vis_out *out, void *stream) {
     if (!scene || !cam || !tile_offsets || !options || !out || width <= 0 || height <= 0) return PF_ERR_ARG;
+    if (!raster_grid_fits(width, height)) return PF_ERR_ARG;
     if (!out->color || !out->depth || !out->normal || !out->alpha || !out->intersections) return PF_ERR_ARG;
     if (scene->face_dtype != PF_F16) return PF_ERR_DTYPE;
     if (scene->num_texel_sites < 1 || scene->num_texel_sites > 16) return PF_ERR_ARG;

@@ -338,6 +338,15 @@ static int check_scene(const pf_scene *s) {
     return PF_OK;
 }
 
+// The recorded backward launches two 256-thread workgroups per four tiles,
+// and a dispatch holds fewer than 2^32 work-items: at most 2^25 tiles of
+// 8x8 rays (2.1 G rays) per call.
+constexpr int64_t kMaxWalkTiles = int64_t(1) << 25;
+
+static bool grid_fits(int32_t B, int32_t W, int32_t H) {
+    return (int64_t)B * ((H + 7) / 8) * ((W + 7) / 8) <= kMaxWalkTiles;
+}
+
 static WalkParams make_params(const pf_scene *scene, const pf_camera *cams, int32_t B, int32_t W, int32_t H,
                               float thr, int32_t max_steps) {
     WalkParams p{};
@@ -360,6 +369,7 @@ extern "C" int pf_walk_forward(const pf_scene *scene, const pf_camera *cams, int
     int rc = check_scene(scene);
     if (rc) return rc;
     if (!cams || num_cams <= 0 || width <= 0 || height <= 0 || !out || !out->color || max_steps <= 0) return PF_ERR_ARG;
+    if (!grid_fits(num_cams, width, height)) return PF_ERR_ARG;
     if (out->seq && out->seq_cap <= 0) return PF_ERR_ARG;
     WalkParams p = make_params(scene, cams, num_cams, width, height, transmittance_threshold, max_steps);
     p.depth_q = depth_quantile;
@@ -383,6 +393,7 @@ extern "C" int pf_walk_backward(const pf_scene *scene, const pf_camera *cams, in
     int rc = check_scene(scene);
     if (rc) return rc;
     if (!cams || num_cams <= 0 || width <= 0 || height <= 0 || !color || !grads || max_steps <= 0) return PF_ERR_ARG;
+    if (!grid_fits(num_cams, width, height)) return PF_ERR_ARG;
     if (!grads->spheres || !grads->nsigmas || !grads->texel_sites || !grads->texel_rgb || !grads->texel_height)
         return PF_ERR_ARG;
     (void)log_t;
@@ -412,6 +423,7 @@ extern "C" int pf_walk_backward_recorded(const pf_scene *scene, const pf_camera 
     int rc = check_scene(scene);
     if (rc) return rc;
     if (!cams || num_cams <= 0 || width <= 0 || height <= 0 || !color || !grads || max_steps <= 0) return PF_ERR_ARG;
+    if (!grid_fits(num_cams, width, height)) return PF_ERR_ARG;
     if (!grads->spheres || !grads->nsigmas || !grads->texel_sites || !grads->texel_rgb || !grads->texel_height)
         return PF_ERR_ARG;
     if (!record || !record->segments || !record->counts || record->cap <= 0 || record->cap % kRecChunk)

@@ -58,3 +58,67 @@ def test_cpu_tensors_rejected_like_the_reference():
 
     with pytest.raises(ValueError):
         _lib.ptr(torch.zeros(3))
+
+
+def test_walk_arguments_rejected_before_any_launch():
+    """Bad sizes come back as PF_ERR_ARG from the host-side checks, before
+    any HIP call: the pointers here are placeholders that are never read."""
+    from powerfoam import _lib
+
+    lib = _lib.load()
+    fake = 256
+    scene = _lib.PfScene(spheres=fake, nsigmas=fake, texel_sites=fake, texel_rgb=fake, texel_height=fake,
+                         adjacency=fake, adjacency_offsets=fake, faces=fake, num_cells=10, num_edges=40,
+                         num_texel_sites=8, attr_dtype=_lib.PF_F32, face_dtype=_lib.PF_F16_PACKED)
+    out = _lib.PfWalkOut(color=fake)
+    fwd = lambda B, W, H, o=out, s=scene: lib.pf_walk_forward(ctypes.byref(s), fake, B, W, H, 1e-4, 0.5, 64,
+                                                             ctypes.byref(o), None)
+    assert fwd(1, 0, 8) == _lib.PF_ERR_ARG
+    assert fwd(1, 8, -1) == _lib.PF_ERR_ARG
+    assert fwd(0, 8, 8) == _lib.PF_ERR_ARG
+    # 2^25 tiles is the largest grid: 4096 cameras of 8192 x 8192 rays exceed it
+    assert fwd(4096, 8192, 8192) == _lib.PF_ERR_ARG
+    bad_dtype = _lib.PfScene.from_buffer_copy(scene)
+    bad_dtype.face_dtype = 7
+    assert fwd(1, 8, 8, s=bad_dtype) == _lib.PF_ERR_DTYPE
+    no_cells = _lib.PfScene.from_buffer_copy(scene)
+    no_cells.num_cells = 0
+    assert fwd(1, 8, 8, s=no_cells) == _lib.PF_ERR_ARG
+    for cap in (0, 12, -8):
+        rec = _lib.PfWalkRecord(segments=fake, counts=fake, cap=cap)
+        o = _lib.PfWalkOut(color=fake, record=ctypes.pointer(rec))
+        assert fwd(1, 8, 8, o=o) == _lib.PF_ERR_ARG, cap
+    grads = _lib.PfGrads(spheres=fake, nsigmas=fake, texel_sites=fake, texel_rgb=fake, texel_height=fake)
+    rec = _lib.PfWalkRecord(segments=fake, counts=fake, cap=20)
+    assert lib.pf_walk_backward_recorded(ctypes.byref(scene), fake, 1, 8, 8, 1e-4, 64, fake, fake, fake, None,
+                                         ctypes.byref(rec), ctypes.byref(grads), 0, None, None,
+                                         None) == _lib.PF_ERR_ARG
+    assert lib.pf_walk_backward(ctypes.byref(scene), fake, 4096, 8192, 8192, 1e-4, 64, fake, fake, fake, None,
+                                ctypes.byref(grads), 0, None, None, None) == _lib.PF_ERR_ARG
+    assert lib.pf_walk_record_slots(2, 9, 17) == 2 * 3 * 2 * 64
+    assert lib.pf_walk_record_slots(0, 9, 17) == 0
+
+
+def test_raster_and_serving_sizes_rejected_before_any_launch():
+    from powerfoam import _lib
+
+    lib = _lib.load()
+    fake = 256
+    scene = _lib.PfScene(spheres=fake, nsigmas=fake, texel_sites=fake, texel_rgb=fake, texel_height=fake,
+                         adjacency=fake, adjacency_offsets=fake, faces=fake, num_cells=10, num_edges=40,
+                         num_texel_sites=8, attr_dtype=_lib.PF_F32, face_dtype=_lib.PF_F16)
+    out = _lib.PfRasterOut(color=fake, log_t=fake, normal_distance=fake, normal=fake, contrib=fake)
+    raster = lambda W, H: lib.pf_raster_forward(ctypes.byref(scene), fake, W, H, fake, fake, fake, 1e-4, None, 0,
+                                                None, 0, 0, ctypes.byref(out), None)
+    # 2^24 tiles of 8x8 is the largest raster image
+    assert raster(65536, 16385) == _lib.PF_ERR_ARG
+    assert raster(0, 8) == _lib.PF_ERR_ARG
+    assert lib.pf_raster_bin_write(fake, 10, fake, (1 << 24) + 1, fake, fake, fake, fake, fake, None, 10,
+                                   None) == _lib.PF_ERR_ARG
+    step = lambda cap, flags=0, rec=None: lib.pf_serve_step(fake, fake, fake, cap, 1e-4, 64, flags, None, rec, None)
+    assert step(0) == _lib.PF_ERR_ARG
+    assert step((1 << 26) + 1) == _lib.PF_ERR_ARG
+    assert step(64, flags=0x10) == _lib.PF_ERR_ARG
+    rec = _lib.PfWalkRecord(segments=fake, counts=fake, cap=100)
+    assert step(64, rec=ctypes.pointer(rec)) == _lib.PF_ERR_ARG
+    assert lib.pf_serve_record_slots(5) == 8 * 64
