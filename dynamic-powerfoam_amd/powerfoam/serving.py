@@ -28,7 +28,7 @@ per-request colour buffer is needed (less memory, more arithmetic).
 Admission: a request reserves its output buffers (its colour buffer, and
 gradient buffers for forward+backward) against ``memory_budget``; it
 waits while the budget is full and is rejected if it waited longer than
-``max_wait`` seconds or can never fit.  ``PlainRenderer`` is the per-request drop-in path used as the
+``max_wait`` seconds after ``submit`` or can never fit.  ``PlainRenderer`` is the per-request drop-in path used as the
 latency baseline: SH colour pre-pass + walk (+ backward), one request at a
 time.
 """
@@ -69,6 +69,7 @@ class RenderRequest:
     rejected: bool = False
     _next_tile: int = 0
     _bytes: int = 0
+    _submitted: float = 0.0
     _cam_dev: torch.Tensor = None
     _texel_rgb: torch.Tensor = None
     _events: list = field(default_factory=list)
@@ -103,7 +104,8 @@ class ResidentScene:
         rgb = texel_rgb if texel_rgb is not None else texel_sites
         self.walk = WalkScene(points, radii, density, normals, texel_sites, rgb, texel_height, adjacency,
                               adjacency_offsets, faces)
-        assert self.walk.c.face_dtype == _lib.PF_F16_PACKED  # the step kernel reads packed faces
+        if self.walk.c.face_dtype != _lib.PF_F16_PACKED:  # the step kernel reads packed faces
+            raise ValueError("served scenes need walk faces that pack to 12-byte records")
         self.points = points.detach().float().contiguous()
         self.radii = radii.detach().float().contiguous()
         self.sh = sh.detach().float().contiguous() if sh is not None else None
@@ -137,6 +139,8 @@ class RenderServer:
                  oldest_share=0.0, streams=3, record_cap=128):
         if colour not in ("prepass", "walk"):
             raise ValueError("colour must be 'prepass' or 'walk'")
+        if not 0 < int(tile_capacity) <= 1 << 26 or int(quantum) <= 0 or int(streams) < 1:
+            raise ValueError("tile_capacity must be in [1, 2^26], quantum positive and streams at least 1")
         self.colour = colour
         self.device = device
         self.budget = int(memory_budget)
@@ -185,6 +189,9 @@ class RenderServer:
 
     # ------------------------------------------------------------ scenes --
     def add_scene(self, name, scene: ResidentScene):
+        if name in self.scenes:
+            # requests and the scene table refer to a resident scene by its index
+            raise ValueError(f"scene {name!r} is already resident")
         need = scene.nbytes()
         if self.reserved + need > self.budget:
             raise MemoryError(f"scene {name} ({need} B) does not fit in the memory budget")
@@ -225,9 +232,16 @@ class RenderServer:
         self._graphs = None  # the launches now carry the workspaces
 
     def submit(self, r: RenderRequest):
+        if r.scene not in self.scenes:
+            raise ValueError(f"scene {r.scene!r} is not resident")
+        if r.mode not in ("forward", "forward_backward"):
+            raise ValueError("mode must be 'forward' or 'forward_backward'")
+        if r.mode == "forward_backward" and (r.target is None or tuple(r.target.shape) != (r.height, r.width, 3)):
+            raise ValueError("a forward_backward request needs an (H, W, 3) target")
         if r.mode == "forward_backward":
             self._ensure_records()
         r._bytes = self._request_bytes(r)
+        r._submitted = time.perf_counter()
         if r._bytes > self.budget - sum(s.nbytes() for s in self.scenes.values()):
             r.rejected = True
             self.rejected.append(r)
@@ -239,7 +253,7 @@ class RenderServer:
         while self.queue and len(self.active) < self.max_reqs:
             r = self.queue[0]
             if self.reserved + r._bytes > self.budget:
-                if now - r.arrival > self.max_wait:
+                if now - r._submitted > self.max_wait:
                     self.queue.popleft()
                     r.rejected = True
                     self.rejected.append(r)
@@ -367,6 +381,11 @@ class RenderServer:
         """Run one step if there is work; returns the number of tiles."""
         now = time.perf_counter() if now is None else now
         self._admit(now)
+        if not self.active and self.queue and self.inflight:
+            # the queue waits for memory that finished steps still hold
+            self.poll(block=True)
+            now = time.perf_counter()
+            self._admit(now)
         if not self.active:
             return 0
         plan = self._schedule()
@@ -436,8 +455,7 @@ class RenderServer:
             while pending and pending[0].arrival <= now:
                 self.submit(pending.popleft())
             if (self.active or self.queue) and len(self.inflight) < max_inflight:
-                if self.step(now) == 0 and self.queue and not self.active:
-                    pass
+                self.step(now)
             else:
                 self.poll(block=bool(self.inflight) and not pending and not self.queue and not self.active)
                 if not self.inflight and not self.active and not self.queue and pending:

@@ -158,6 +158,51 @@ def test_memory_admission_rejects_what_cannot_fit(served):
     assert small.done is not None
 
 
+def test_request_waits_for_memory_held_by_a_finished_step(served):
+    """Budget for one request at a time: the second, submitted directly
+    (arrival left at 0), waits for the first to release its memory and is
+    then served; max_wait counts from submit."""
+    prepass = served.num_cells * served.K * 12
+    srv = RenderServer(DEV, memory_budget=served.nbytes() + 64 * 64 * 16 + prepass + 1024, max_wait=30.0)
+    srv.add_scene("s", served)
+    a, b = (RenderRequest(scene="s", camera=workloads.front_camera(64, 64), rid=k) for k in range(2))
+    assert srv.submit(a) and srv.submit(b)
+    _drain(srv, [])
+    assert not srv.rejected and a.done is not None and b.done is not None
+    assert b.admitted >= a.done
+    assert torch.equal(a.color, b.color)
+    assert srv.reserved == served.nbytes()
+
+
+def test_bad_scenes_and_requests_are_refused(served):
+    """A second scene under a resident name, an unknown scene or mode, or a
+    forward+backward request without a matching target: ValueError at
+    add_scene / submit, before anything reaches the request table."""
+    with pytest.raises(ValueError):
+        RenderServer(DEV, tile_capacity=0)
+    with pytest.raises(ValueError):
+        RenderServer(DEV, tile_capacity=(1 << 26) + 1)
+    srv = RenderServer(DEV)
+    srv.add_scene("s", served)
+    reserved = srv.reserved
+    with pytest.raises(ValueError):
+        srv.add_scene("s", served)
+    assert srv.reserved == reserved and srv.scene_index == {"s": 0}
+    cam = workloads.front_camera(32, 24)
+    with pytest.raises(ValueError):
+        srv.submit(RenderRequest(scene="nope", camera=cam))
+    with pytest.raises(ValueError):
+        srv.submit(RenderRequest(scene="s", camera=cam, mode="backward"))
+    with pytest.raises(ValueError):
+        srv.submit(RenderRequest(scene="s", camera=cam, mode="forward_backward"))
+    with pytest.raises(ValueError):
+        srv.submit(RenderRequest(scene="s", camera=cam, mode="forward_backward", target=torch.zeros(32, 24, 3)))
+    assert not srv.queue
+    ok = RenderRequest(scene="s", camera=cam, mode="forward_backward", target=torch.zeros(24, 32, 3))
+    _drain(srv, [ok])
+    assert ok.done is not None and int(srv.status.item()) & ~4 == 0
+
+
 def test_prepass_colour_equals_walk_colour(served):
     """Texel colours from the per-request pre-pass and SH evaluated inside
     the walk are the same numbers, so the images are identical."""
