@@ -178,6 +178,15 @@ class Rasterizer:
         perr = torch.zeros(N, dtype=torch.float32, device=dev) if return_point_err else None
         err = torch.zeros(H, W, dtype=torch.float32, device=dev) if self.loss_accumulation else None
         gt = _c(ray_gt.detach(), torch.float32) if ray_gt is not None else None
+        # the kernels index these per cell and per pixel
+        K = sites.shape[1] if sites.dim() == 3 else -1
+        if tuple(sites.shape) != (N, K, 3) or tuple(rgb.shape) != (N, K, 3) or tuple(height.shape) != (N, K) or \
+                off.numel() != N + 1:
+            raise ValueError("texel_sites / texel_rgb must be (N, K, 3), texel_height (N, K), offsets N + 1")
+        if dq is not None and (dq.dim() < 1 or dq.shape[-1] < 1 or dq.numel() != H * W * dq.shape[-1]):
+            raise ValueError("depth_quantiles must be (H, W, Q)")
+        if gt is not None and gt.numel() != H * W * 3:
+            raise ValueError("ray_gt must hold (H, W, 3) values")
         tiles = offsets.numel() - 1
         early = torch.zeros(tiles, dtype=torch.int32, device=dev)
         out = _lib.PfRasterOut()

@@ -52,6 +52,8 @@ class WalkScene:
         if self.texel_sites.dim() != 3 or self.texel_sites.shape[0] != n:
             raise ValueError("texel_sites must be (N, K, 3)")
         self.K = self.texel_sites.shape[1]
+        if tuple(self.texel_rgb.shape[-3:]) != (n, self.K, 3) or tuple(self.texel_height.shape) != (n, self.K):
+            raise ValueError("texel_rgb must be (N, K, 3) and texel_height (N, K)")
         if adjacency.dtype not in (torch.int32, torch.int64) or adjacency_offsets.dtype not in (torch.int32,
                                                                                                 torch.int64):
             raise ValueError("adjacency / adjacency_offsets must be integer tensors")
@@ -179,6 +181,17 @@ def _record_struct(record):
     return r
 
 
+def _expect(t, shape, name, dtype=torch.float32, device=None):
+    """The kernels index these buffers by the launch's sizes: refuse any
+    other shape, dtype, device or layout instead of reading past them."""
+    if t is None:
+        return
+    if tuple(t.shape) != tuple(shape) or t.dtype != dtype or not t.is_contiguous() or \
+            (device is not None and t.device != device):
+        raise ValueError(f"{name} must be a contiguous {dtype} tensor of shape {tuple(shape)} on {device}, "
+                         f"got {t.dtype} {tuple(t.shape)} on {t.device}")
+
+
 def launch_walk_forward(scene, cams_dev, width, height, transmittance_threshold=1e-3, depth_quantile=0.5,
                         max_steps=DEFAULT_MAX_STEPS, want_log_t=True, want_depth=True, want_sequences=False,
                         seq_cap=0, want_bytes=False, record_cap=0):
@@ -188,6 +201,7 @@ def launch_walk_forward(scene, cams_dev, width, height, transmittance_threshold=
     lib = _lib.load()
     B = cams_dev.shape[0]
     dev = cams_dev.device
+    _expect(cams_dev, (B, 12), "cams_dev", dtype=torch.int32)
     color = torch.empty(B, height, width, 3, dtype=torch.float32, device=dev)
     log_t = torch.empty(B, height, width, dtype=torch.float32, device=dev) if want_log_t else None
     depth = torch.empty(B, height, width, dtype=torch.float32, device=dev) if want_depth else None
@@ -230,6 +244,12 @@ def launch_walk_backward(scene, cams_dev, width, height, color, log_t, grad_colo
     lib = _lib.load()
     n, K = scene.num_cells, scene.K
     dev = cams_dev.device
+    B = cams_dev.shape[0]
+    _expect(cams_dev, (B, 12), "cams_dev", dtype=torch.int32)
+    _expect(color, (B, height, width, 3), "color", device=dev)
+    _expect(log_t, (B, height, width), "log_t", device=dev)
+    _expect(grad_color, (B, height, width, 3), "grad_color", device=dev)
+    _expect(grad_log_t, (B, height, width), "grad_log_t", device=dev)
     if grads is None:
         grads = dict(
             spheres=torch.zeros(n, 4, dtype=torch.float32, device=dev),
@@ -238,6 +258,16 @@ def launch_walk_backward(scene, cams_dev, width, height, color, log_t, grad_colo
             texel_rgb=torch.zeros(n, K, 3, dtype=torch.float32, device=dev),
             texel_height=torch.zeros(n, K, dtype=torch.float32, device=dev),
         )
+    for k, shape in (("spheres", (n, 4)), ("nsigmas", (n, 4)), ("texel_sites", (n, K, 3)),
+                     ("texel_rgb", (n, K, 3)), ("texel_height", (n, K))):
+        t = grads[k]
+        if k == "texel_rgb" and texel_rgb_cam_stride:
+            # camera b's rows start texel_rgb_cam_stride floats after camera b - 1's
+            need = (B - 1) * texel_rgb_cam_stride + 3 * K * n
+            if t.dtype != torch.float32 or not t.is_contiguous() or t.device != dev or t.numel() < need:
+                raise ValueError(f"grads['texel_rgb'] must hold {B} cameras' rows at stride {texel_rgb_cam_stride}")
+        else:
+            _expect(t, shape, f"grads[{k!r}]", device=dev)
     g = _lib.PfGrads()
     for k in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
         setattr(g, k, _lib.ptr(grads[k]))

@@ -221,3 +221,25 @@ def test_rasterizer_error_conventions(scene4k, cech4k):
     assert out[5] is not None
     with pytest.raises(NotImplementedError):  # rasterize.py:1232-1234
         out[0].sum().backward()
+
+
+def test_mis_shaped_inputs_are_refused(scene_textured):
+    """Per-pixel and per-cell inputs whose shapes do not match the camera or
+    the scene raise ValueError instead of being read past their ends."""
+    from powerfoam import raytrace as rt
+
+    sc = scene_textured
+    adj, off = orc.cech(sc["points"], sc["radii"])
+    cam = workloads.front_camera(24, 16)
+    with pytest.raises(ValueError, match="depth_quantiles"):
+        _rasterize_both(sc, adj, off, cam, depth_q=np.full((16, 12, 2), 0.5, np.float32))
+    with pytest.raises(ValueError, match="ray_gt"):
+        _rasterize_both(sc, adj, off, cam, ray_gt=np.zeros((16, 23, 3), np.float32), point_err=True)
+    short = dict(sc, texel_rgb=np.ascontiguousarray(sc["texel_rgb"][:-1]))
+    with pytest.raises(ValueError, match="texel_rgb"):
+        _rasterize_both(short, adj, off, cam)
+    p = {k: t(sc[k]) for k in ("points", "radii", "density", "normals", "texel_sites", "texel_rgb", "texel_height")}
+    with pytest.raises(ValueError, match="texel_height"):
+        rt.WalkScene(p["points"], p["radii"], p["density"], p["normals"], p["texel_sites"], p["texel_rgb"],
+                     p["texel_height"][:, :4], t(sc["adjacency"]), t(sc["offsets"]),
+                     rt.walk_faces(p["points"], p["radii"], t(sc["adjacency"]).int(), t(sc["offsets"]).int()))

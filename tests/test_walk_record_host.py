@@ -53,3 +53,40 @@ def test_record_cap_fits_budget():
     assert rt.RayTracer(types.SimpleNamespace(walk_record_cap=0), "cpu").record_cap_for(1, 64, 64) == 0
     # the default: 128 segments per ray, 4.2 GB for one 1080p frame
     assert rt.RayTracer(None, "cpu").record_cap_for(1, 1920, 1080) == rt.DEFAULT_RECORD_CAP == 128
+
+
+def test_walk_launch_buffers_are_checked_before_any_launch():
+    """Wrong shapes, dtypes or layouts of the camera words, checkpoint,
+    incoming gradients or gradient buffers raise ValueError on the host
+    (CPU tensors: the check comes before any pointer reaches the library)."""
+    import types
+
+    import pytest
+    import torch
+
+    from powerfoam import raytrace as rt
+
+    scene = types.SimpleNamespace(num_cells=10, K=8)
+    B, W, H = 2, 16, 8
+    cams = torch.zeros(B, 12, dtype=torch.int32)
+    color = torch.zeros(B, H, W, 3)
+    log_t = torch.zeros(B, H, W)
+    bwd = lambda **kw: rt.launch_walk_backward(scene, kw.pop("cams", cams), W, H, kw.pop("color", color), log_t,
+                                               kw.pop("gc", color), None, **kw)
+    with pytest.raises(ValueError, match="cams_dev"):
+        bwd(cams=torch.zeros(B, 12))
+    with pytest.raises(ValueError, match="color"):
+        bwd(color=torch.zeros(B, W, H, 3))
+    with pytest.raises(ValueError, match="grad_color"):
+        bwd(gc=torch.zeros(B, H, W, 3).transpose(1, 2).contiguous().transpose(1, 2))
+    good = dict(spheres=torch.zeros(10, 4), nsigmas=torch.zeros(10, 4), texel_sites=torch.zeros(10, 8, 3),
+                texel_rgb=torch.zeros(10, 8, 3), texel_height=torch.zeros(10, 8))
+    for k in good:
+        bad = dict(good)
+        bad[k] = torch.zeros(9, *good[k].shape[1:])
+        with pytest.raises(ValueError, match=k):
+            bwd(grads=bad)
+    with pytest.raises(ValueError, match="stride"):
+        bwd(grads=good, texel_rgb_cam_stride=240)  # two cameras' rows need 480 floats
+    with pytest.raises(ValueError, match="cams_dev"):
+        rt.launch_walk_forward(scene, torch.zeros(B, 12), W, H)
