@@ -90,3 +90,25 @@ def test_walk_launch_buffers_are_checked_before_any_launch():
         bwd(grads=good, texel_rgb_cam_stride=240)  # two cameras' rows need 480 floats
     with pytest.raises(ValueError, match="cams_dev"):
         rt.launch_walk_forward(scene, torch.zeros(B, 12), W, H)
+
+
+def test_sh_prepass_buffers_are_checked_before_any_launch():
+    import pytest
+    import torch
+
+    from powerfoam import shading
+
+    coeffs, sites = torch.zeros(10, 16, 3), torch.zeros(10, 8, 3)
+    cams = torch.zeros(2, 12, dtype=torch.int32)
+    with pytest.raises(ValueError, match="coeffs"):
+        shading.sh_forward(torch.zeros(10, 9, 3), 3, sites, cams)
+    with pytest.raises(ValueError, match="texel_sites"):
+        shading.sh_forward(coeffs, 3, torch.zeros(10, 8, 2), cams)
+    with pytest.raises(ValueError, match="cams_dev"):
+        shading.sh_forward(coeffs, 3, sites, torch.zeros(2, 12))
+    with pytest.raises(ValueError, match="out"):
+        shading.sh_forward(coeffs, 3, sites, cams, out=torch.zeros(1, 10, 8, 3))
+    with pytest.raises(ValueError, match="grad_rgb"):
+        shading.sh_backward(coeffs, 3, sites, cams, torch.zeros(1, 10, 8, 3))
+    with pytest.raises(ValueError, match="grad_coeffs"):
+        shading.sh_backward(coeffs, 3, sites, cams, torch.zeros(2, 10, 8, 3), grad_coeffs=torch.zeros(10, 9, 3))
