@@ -73,6 +73,11 @@ def build_power_adjacency_gpu(points, radii, margin=None, sites_per_cell=3.0, fa
     pts = points.detach().to(torch.float32).contiguous()
     rad = radii.detach().to(torch.float32).contiguous()
     N = pts.shape[0]
+    if pts.dim() != 2 or pts.shape[1] != 3 or tuple(rad.shape) != (N,) or rad.device != dev:
+        raise ValueError("points must be (N, 3) and radii (N,) on one device")
+    # the bucket grid is sized from the sites' bounds
+    if N and not bool(torch.isfinite(pts).all() and torch.isfinite(rad).all()):
+        raise ValueError("points and radii must be finite")
     if N < 2:
         return torch.zeros(0, dtype=torch.int32, device=dev), torch.zeros(N + 1, dtype=torch.int32, device=dev)
     L = lifted_coordinate(pts, rad)

@@ -66,3 +66,15 @@ def test_walks_agree(scene4k):
 def test_rejects_host_tensors():
     with pytest.raises(ValueError):
         build_power_adjacency_gpu(torch.zeros(4, 3), torch.zeros(4))
+
+
+def test_rejects_mismatched_or_non_finite_sites():
+    pts = torch.rand(64, 3, device=DEV)
+    with pytest.raises(ValueError, match="radii"):
+        build_power_adjacency_gpu(pts, torch.full((63,), 0.01, device=DEV))
+    with pytest.raises(ValueError, match="points"):
+        build_power_adjacency_gpu(pts[:, :2].contiguous(), torch.full((64,), 0.01, device=DEV))
+    bad = pts.clone()
+    bad[5, 1] = float("nan")
+    with pytest.raises(ValueError, match="finite"):
+        build_power_adjacency_gpu(bad, torch.full((64,), 0.01, device=DEV))
