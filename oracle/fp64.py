@@ -141,6 +141,89 @@ def walk_ray(o, d, scene, thr=1e-3, depth_q=0.5):
     return rgb, log_t, depth, visited, margin
 
 
+def raster_ray(o, d, scene, quantiles=(), gt=None, thr=1e-7):
+    """FP64 side outputs of the tile rasterizer's forward for one ray
+    (rasterize.py:1088-1129): colour, log T, normal distance
+    sum_{n.d > 0} (n.d)^2 alpha T, normal sum alpha T n, the depths where T
+    first drops below each quantile (quantiles descending; -1 where never
+    reached), and per cell the terms alpha T (contrib before the 1/(H W)
+    scale) and alpha T |c - gt|_1 (point_err before the scale).  Cells come
+    in envelope order, which is the power-distance order of the raster's
+    tile lists (along a ray the envelope owners' power distances from the
+    eye increase); a tiny threshold makes the raster's per-tile stop
+    irrelevant.  No sigma skip: the raster composites every hit."""
+    seq, margin = owner_sequence(o, d, scene["points"], scene["radii"])
+    rgb = np.zeros(3)
+    normal = np.zeros(3)
+    nd = 0.0
+    log_t = 0.0
+    qd = np.full(len(quantiles), -1.0)
+    qi = 0
+    cells, contrib, perr = [], [], []
+    for cell, t_in, t_out in seq:
+        trans = np.exp(log_t)
+        if trans < thr:
+            break
+        c = scene["points"][cell].astype(np.float64)
+        r = float(scene["radii"][cell])
+        hit, tn, tf = sphere_chord(o, d, c, r)
+        if np.linalg.norm(c - o) < 4.0 * r:
+            hit = False
+        tn, tf = max(tn, t_in), min(tf, t_out)
+        if not hit or tn > tf:
+            continue
+        sigma = float(scene["density"][cell])
+        n = scene["normals"][cell].astype(np.float64)
+        ts, dp, col = texel(o, d, tn, c, n, r, scene["texel_sites"][cell].astype(np.float64),
+                            scene["texel_rgb"][cell].astype(np.float64),
+                            scene["texel_height"][cell].astype(np.float64))
+        if dp >= 0:
+            tf = min(ts, tf)
+        else:
+            tn = max(ts, tn)
+        dt = tf - tn
+        if dt <= 0:
+            continue
+        alpha = -np.expm1(-sigma * dt)
+        w = alpha * trans
+        rgb += col * w
+        log_t -= sigma * dt
+        ndv = float(n @ d)
+        if ndv > 0:
+            nd += ndv * ndv * w
+        normal += w * n
+        cells.append(cell)
+        contrib.append(w)
+        perr.append(w * (np.abs(col - gt).sum() if gt is not None else 0.0))
+        nxt = np.exp(log_t)
+        while qi < len(quantiles) and nxt < quantiles[qi]:
+            qd[qi] = tn + np.log(trans / quantiles[qi]) / sigma
+            qi += 1
+    return dict(color=rgb, log_t=log_t, normal_distance=nd, normal=normal, qdepth=qd, cells=cells,
+                contrib=np.array(contrib), point_err=np.array(perr), margin=margin)
+
+
+def raster_render(scene, cam_words, quantiles=(), ray_gt=None, thr=1e-7, pixels=None):
+    """FP64 raster side outputs over the given flat pixel ids (default all);
+    contrib and point_err are per cell and carry the 1/(H W) scale."""
+    eye, dirs = camera_rays(cam_words)
+    w = np.asarray(cam_words, np.int32)
+    H, W = int(w[11]), int(w[10])
+    ids = np.arange(dirs.shape[0]) if pixels is None else np.asarray(pixels)
+    n = len(scene["radii"])
+    out = dict(color=np.zeros((ids.size, 3)), log_t=np.zeros(ids.size), normal_distance=np.zeros(ids.size),
+               normal=np.zeros((ids.size, 3)), qdepth=np.zeros((ids.size, len(quantiles))),
+               contrib=np.zeros(n), point_err=np.zeros(n), margin=np.zeros(ids.size))
+    for k, pid in enumerate(ids):
+        gt = None if ray_gt is None else np.asarray(ray_gt, np.float64).reshape(-1, 3)[pid]
+        r = raster_ray(eye, dirs[pid], scene, quantiles, gt, thr)
+        for key in ("color", "log_t", "normal_distance", "normal", "qdepth", "margin"):
+            out[key][k] = r[key]
+        np.add.at(out["contrib"], r["cells"], r["contrib"] / (H * W))
+        np.add.at(out["point_err"], r["cells"], r["point_err"] / (H * W))
+    return out
+
+
 def render(scene, cam_words, thr=1e-3, depth_q=0.5, pixels=None):
     """FP64 render of the given flat pixel ids (default: all)."""
     eye, dirs = camera_rays(cam_words)

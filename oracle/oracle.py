@@ -232,7 +232,10 @@ def _rlib():
         lib.orc_raster_forward.argtypes = ([c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_int32,
                                             c_void_p, c_int32, c_int32] + [c_void_p] * 9 + [c_int32])
         lib.orc_raster_backward.argtypes = ([c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32]
-                                            + [c_void_p] * 12 + [c_int32])
+                                            + [c_void_p] * 12 + [c_int32, c_int32])
+        lib.orc_raster_loss_f64.restype = c_double
+        lib.orc_raster_loss_f64.argtypes = ([c_void_p] * 6 + [c_int32, c_int64, c_void_p, c_void_p, c_int64,
+                                             c_void_p, c_double, c_void_p, c_double, c_void_p, c_int32])
         lib.orc_raster_plain.argtypes = ([c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_int32, c_int32,
                                           c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                           c_int32])
@@ -307,7 +310,9 @@ def raster_forward(scene, cam_words, offsets, prims, thr=1e-3, depth_q=None, ray
 
 
 def raster_backward(scene, cam_words, offsets, prims, early, log_t, g_color, g_log_t=None, g_ndist=None,
-                    g_normal=None, depth_q=None, g_qdepth=None, g_contrib=None, nthreads=None):
+                    g_normal=None, depth_q=None, g_qdepth=None, g_contrib=None, exact=False, nthreads=None):
+    """exact: keep every texel and the radius term of the texel weights (the
+    reference drops both, texture.py:118-119, :158-159)."""
     lib = _rlib()
     cam = _c(cam_words, np.int32).reshape(12)
     n, K = scene.N, scene.K
@@ -320,8 +325,36 @@ def raster_backward(scene, cam_words, offsets, prims, early, log_t, g_color, g_l
     lib.orc_raster_backward(ctypes.byref(scene.c), _p(cam), _p(_c(offsets, np.int64)), _p(_c(prims, np.int32)),
                             _p(_c(early, np.int32)), _p(args[0]), Q, *[_p(a) for a in args[1:]],
                             _p(g["spheres"]), _p(g["nsigmas"]), _p(g["texel_sites"]), _p(g["texel_rgb"]),
-                            _p(g["texel_height"]), nthreads or num_threads())
+                            _p(g["texel_height"]), 1 if exact else 0, nthreads or num_threads())
     return g
+
+
+class _RasterWeights(ctypes.Structure):
+    _fields_ = [("g_nd", c_void_p), ("nd_pow", c_int32), ("g_normal", c_void_p), ("quantiles", c_void_p),
+                ("g_q", c_void_p), ("Q", c_int32), ("g_contrib", c_void_p)]
+
+
+def raster_loss_f64(params, adjacency, offsets, cam_words, target, w_alpha=0.0, g_ndist=None, nd_pow=2,
+                    g_normal=None, depth_q=None, g_qdepth=None, g_contrib=None, thr=1e-7, nthreads=None):
+    """FP64 value of sum_px [|C - target|^2 + w_alpha (1 - T) + g_ndist nd
+    + g_normal . normal + sum_q g_qdepth_q depth_q] + sum_cells g_contrib
+    contrib, the raster's outputs evaluated along the exact-face walk of
+    the power CSR (one camera).  nd is sum_{n.d>0} (n.d)^nd_pow alpha T."""
+    lib = _rlib()
+    P = {k: _c(v, np.float64) for k, v in params.items()}
+    adjacency, offsets = _c(adjacency, np.int32), _c(offsets, np.int32)
+    cam = _c(cam_words, np.int32).reshape(12)
+    d64 = lambda a: None if a is None else _c(a, np.float64)
+    keep = [d64(g_ndist), d64(g_normal), None if depth_q is None else _c(depth_q, np.float32), d64(g_qdepth),
+            d64(g_contrib)]
+    rw = _RasterWeights(_p(keep[0]), nd_pow, _p(keep[1]), _p(keep[2]), _p(keep[3]),
+                        0 if depth_q is None else depth_q.shape[-1], _p(keep[4]))
+    K = P["texel_sites"].shape[1]
+    N = P["points"].shape[0]
+    return lib.orc_raster_loss_f64(_p(P["points"]), _p(P["radii"]), _p(P["nsigmas"]), _p(P["texel_sites"]),
+                                   _p(P["texel_rgb"]), _p(P["texel_height"]), K, N, _p(adjacency), _p(offsets),
+                                   adjacency.size, _p(cam), thr, _p(_c(target, np.float32)), w_alpha,
+                                   ctypes.byref(rw), nthreads or num_threads())
 
 
 def raster_plain(scene, cam_words, offsets, prims, thr=1e-3, vis=False, max_isect=1024, depth_qv=0.5,

@@ -413,11 +413,43 @@ template <typename R> struct Brent {
     }
 };
 
+// Side outputs of the tile rasterizer's forward (rasterize.py:1088-1129)
+// as one weighted sum, evaluated along a walk: normal distance
+// sum_{n.d > 0} (n.d)^P alpha T, normal sum alpha T n, per-cell contrib
+// alpha T / (H W), and the depth where T first drops below each quantile
+// (quantiles descending).  Along a ray the walk visits cells in the order
+// of their power distance from the eye, which is the order of the raster's
+// tile lists, so the FP64 walk evaluates the raster's forward.
+template <typename R> struct RasterTerms {
+    R g_nd = 0;
+    int nd_pow = 2;
+    V3<R> g_n{0, 0, 0};
+    const float *q = nullptr;
+    const double *g_q = nullptr;
+    int Q = 0, qi = 0;
+    const double *g_contrib = nullptr;
+    R inv_hw = 0;
+    R value = 0;
+    void segment(int prim, R tn, R alpha, R trans, R sigma, V3<R> n, V3<R> d, R log_t_after) {
+        const R w = alpha * trans;
+        const R ndv = dot(n, d);
+        if (ndv > R(0)) value += g_nd * (nd_pow == 2 ? ndv * ndv : ndv) * w;
+        value += dot(g_n, n) * w;
+        if (g_contrib) value += R(g_contrib[prim]) * w * inv_hw;
+        const R next = xexp(log_t_after);
+        while (qi < Q && next < R(q[qi])) {
+            value += R(g_q[qi]) * (tn + xlog(trans / R(q[qi])) / sigma);
+            ++qi;
+        }
+    }
+};
+
 // The walk: raytrace.py:115-241 (loop :163-236, face loop :183-200,
 // texel :213-226, compositing :230-236); depth as rasterize.py:871-877.
+// terms: NULL, or raster side outputs to accumulate per composited segment.
 template <typename R, typename FaceLoad>
 RayOut<R> walk_ray(const SceneR<R> &s, FaceLoad face, V3<R> o, V3<R> d, int start, R thr, R depth_q, int max_steps,
-                   int32_t *seq, int seq_cap) {
+                   int32_t *seq, int seq_cap, RasterTerms<R> *terms = nullptr) {
     RayOut<R> out{mk<R>(0, 0, 0), R(0), R(0), 0, kFnvOffset, 0};
     R log_t = 0;
     bool found = false;
@@ -458,6 +490,7 @@ RayOut<R> walk_ray(const SceneR<R> &s, FaceLoad face, V3<R> o, V3<R> d, int star
         const Tex<R> tx = texel_fwd(o, d, tn, c, nrm, r, s.sites + 3 * tb, s.rgb + 3 * tb, s.height + tb, s.K);
         tf = tx.dp >= R(0) ? mn(tx.t_surf, tf) : tf;
         tn = tx.dp < R(0) ? mx(tx.t_surf, tn) : tn;
+        const int prim_here = prim;
         prim = next;
         const R dt = tf - tn;
         if (dt > R(0)) {
@@ -471,6 +504,7 @@ RayOut<R> walk_ray(const SceneR<R> &s, FaceLoad face, V3<R> o, V3<R> d, int star
                 out.depth = tn + xlog(trans / depth_q) / sigma;
                 found = true;
             }
+            if (terms) terms->segment(prim_here, tn, alpha, trans, sigma, nrm, d, log_t);
         }
     }
     out.log_t = log_t;
@@ -754,13 +788,25 @@ int orc_walk_backward(const SceneIn *scene, const CamIn *cams, int32_t B, float 
 
 // FP64 forward for finite differences: parameters in double, walk faces
 // computed exactly from points/radii (benchmark.py:297-301 without the fp16
-// cast).  Returns sum over rays of the per-ray loss
-//   sum_c w_c (C_c - target_c)^2 + w_alpha * (1 - exp(log_t))
-// restricted to the given rays (NULL = all), and writes per-ray colour.
-double orc_walk_loss_f64(const double *points, const double *radii, const double *nsigmas, const double *sites,
-                         const double *rgb, const double *height, int32_t K, int64_t N, const int32_t *adj,
-                         const int32_t *off, int64_t E, const CamIn *cams, int32_t B, double thr, int32_t max_steps,
-                         const int64_t *rays, int64_t n_rays, const float *target, double w_alpha, int32_t nthreads) {
+// cast).  Sum over rays of the per-ray loss
+//   sum_c (C_c - target_c)^2 + w_alpha * (1 - exp(log_t)) [+ raster terms]
+// restricted to the given rays (NULL = all).  rt: NULL, or the weights of
+// the raster side outputs (one camera; per-pixel arrays (H,W[,3|,Q])).
+struct RasterWeights {
+    const double *g_nd;      // (H,W) or NULL
+    int32_t nd_pow;          // 2 forward definition, 1 the reference backward's
+    const double *g_normal;  // (H,W,3) or NULL
+    const float *quantiles;  // (H,W,Q) descending, or NULL
+    const double *g_q;       // (H,W,Q)
+    int32_t Q;
+    const double *g_contrib; // (N,) or NULL
+};
+
+static double loss_f64(const double *points, const double *radii, const double *nsigmas, const double *sites,
+                       const double *rgb, const double *height, int32_t K, int64_t N, const int32_t *adj,
+                       const int32_t *off, int64_t E, const CamIn *cams, int32_t B, double thr, int32_t max_steps,
+                       const int64_t *rays, int64_t n_rays, const float *target, double w_alpha,
+                       const RasterWeights *rw, int32_t nthreads) {
     std::vector<double> sph(4 * N), faces(4 * E), pm(N);
     for (int64_t i = 0; i < N; ++i) {
         sph[4 * i] = points[3 * i];
@@ -791,12 +837,47 @@ double orc_walk_loss_f64(const double *points, const double *radii, const double
         const int j = (int)(ray % W);
         const Cam<double> c = make_cam<double>(cams[b]);
         const V3<double> d = ray_dir(c, i, j);
-        const RayOut<double> r = walk_ray(s, fl, c.eye, d, c.start, thr, 0.5, max_steps, (int32_t *)nullptr, 0);
+        RasterTerms<double> tm;
+        if (rw) {
+            const int64_t px = (int64_t)i * W + j;
+            tm.g_nd = rw->g_nd ? rw->g_nd[px] : 0.0;
+            tm.nd_pow = rw->nd_pow;
+            if (rw->g_normal) tm.g_n = mk(rw->g_normal[3 * px], rw->g_normal[3 * px + 1], rw->g_normal[3 * px + 2]);
+            if (rw->quantiles) {
+                tm.q = rw->quantiles + px * rw->Q;
+                tm.g_q = rw->g_q + px * rw->Q;
+                tm.Q = rw->Q;
+            }
+            tm.g_contrib = rw->g_contrib;
+            tm.inv_hw = 1.0 / ((double)H * W);
+        }
+        const RayOut<double> r = walk_ray(s, fl, c.eye, d, c.start, thr, 0.5, max_steps, (int32_t *)nullptr, 0,
+                                          rw ? &tm : nullptr);
         const double e0 = r.rgb.x - target[3 * ray], e1 = r.rgb.y - target[3 * ray + 1],
                      e2 = r.rgb.z - target[3 * ray + 2];
-        loss += e0 * e0 + e1 * e1 + e2 * e2 + w_alpha * (1.0 - std::exp(r.log_t));
+        loss += e0 * e0 + e1 * e1 + e2 * e2 + w_alpha * (1.0 - std::exp(r.log_t)) + tm.value;
     }
     return loss;
+}
+
+double orc_walk_loss_f64(const double *points, const double *radii, const double *nsigmas, const double *sites,
+                         const double *rgb, const double *height, int32_t K, int64_t N, const int32_t *adj,
+                         const int32_t *off, int64_t E, const CamIn *cams, int32_t B, double thr, int32_t max_steps,
+                         const int64_t *rays, int64_t n_rays, const float *target, double w_alpha, int32_t nthreads) {
+    return loss_f64(points, radii, nsigmas, sites, rgb, height, K, N, adj, off, E, cams, B, thr, max_steps, rays,
+                    n_rays, target, w_alpha, nullptr, nthreads);
+}
+
+// The same loss plus the raster side outputs weighted by rw (one camera):
+// the functional whose gradient the raster backward computes from incoming
+// gradients (color, log_t, normal_distance, normal, quantile depths,
+// contrib; rasterize.py:1371-1426).
+double orc_raster_loss_f64(const double *points, const double *radii, const double *nsigmas, const double *sites,
+                           const double *rgb, const double *height, int32_t K, int64_t N, const int32_t *adj,
+                           const int32_t *off, int64_t E, const CamIn *cam, double thr, const float *target,
+                           double w_alpha, const RasterWeights *rw, int32_t nthreads) {
+    return loss_f64(points, radii, nsigmas, sites, rgb, height, K, N, adj, off, E, cam, 1, thr, 1 << 16, nullptr, 0,
+                    target, w_alpha, rw, nthreads);
 }
 
 }  // extern "C"
@@ -930,10 +1011,14 @@ RTex rtex_fwd(V3<float> o, V3<float> d, float tn, V3<float> c, V3<float> n, floa
     return t;
 }
 
-// texture.py:74-186 (raster weights); texel grads into red (layout as tex)
+// texture.py:74-186 (raster weights); texel grads into red (layout as tex).
+// exact == false reproduces the reference's two approximations (texels under
+// 1e-3 of the weight sum skipped, texture.py:118-119, :158-159; no radius
+// term through the weights); exact == true keeps every texel and adds the
+// radius term d w / d r = w * 20 |q - s|^2 / r^3 into dr.
 void rtex_bwd(V3<float> o, V3<float> d, float tn_in, V3<float> c, V3<float> n, float r, bool hit, const float *tex,
               int K, const RTex &fw, V3<float> g_rgb, float dts_in, float *red, V3<float> &dc, V3<float> &dn,
-              float &dtn) {
+              float &dtn, bool exact, float &dr) {
     dc = dn = mk<float>(0, 0, 0);
     dtn = 0.0f;
     if (!hit) return;
@@ -946,7 +1031,7 @@ void rtex_bwd(V3<float> o, V3<float> d, float tn_in, V3<float> c, V3<float> n, f
     for (int i = 0; i < K; ++i) {
         const V3<float> v = sub(q, mk(tex[3 * i], tex[3 * i + 1], tex[3 * i + 2]));
         const float w = orc_expf(-10.0f * (dot(v, v) / r2));
-        if (w / fw.wsum < 1e-3f) continue;
+        if (!exact && w / fw.wsum < 1e-3f) continue;
         const V3<float> dwds = dv(scl((2.0f * 10.0f) * w, v), r2);
         const V3<float> grgb = dv(scl(w, g_rgb), fw.wsum);
         red[3 * K + 3 * i] += grgb.x;
@@ -958,6 +1043,7 @@ void rtex_bwd(V3<float> o, V3<float> d, float tn_in, V3<float> c, V3<float> n, f
         red[3 * i + 1] += gs.y;
         red[3 * i + 2] += gs.z;
         dq = add(dq, scl(-1.0f, gs));
+        if (exact) dr += ((gdp - gdp_in) / fw.wsum) * (((20.0f * w) * dot(v, v)) / (r2 * r));
     }
     const float dtq = dot(dq, d);
     dtn = (fw.dp >= 0.0f || tn_in >= fw.t_surf) ? dtq : 0.0f;
@@ -978,7 +1064,7 @@ void rtex_bwd(V3<float> o, V3<float> d, float tn_in, V3<float> c, V3<float> n, f
     for (int i = 0; i < K; ++i) {
         const V3<float> v = sub(q0, mk(tex[3 * i], tex[3 * i + 1], tex[3 * i + 2]));
         const float w = orc_expf(-10.0f * (dot(v, v) / r2));
-        if (w / fw.wsum0 < 1e-3f) continue;
+        if (!exact && w / fw.wsum0 < 1e-3f) continue;
         const V3<float> dwds = dv(scl((2.0f * 10.0f) * w, v), r2);
         red[6 * K + i] += (w * dh) / fw.wsum0;
         const float hdp = dh * tex[6 * K + i];
@@ -987,6 +1073,7 @@ void rtex_bwd(V3<float> o, V3<float> d, float tn_in, V3<float> c, V3<float> n, f
         red[3 * i + 1] += gs.y;
         red[3 * i + 2] += gs.z;
         dq0 = add(dq0, scl(-1.0f, gs));
+        if (exact) dr += ((hdp - hdp_in) / fw.wsum0) * (((20.0f * w) * dot(v, v)) / (r2 * r));
     }
     const float dtq0 = dot(dq0, d);
     dtn += (fw.dp >= 0.0f || tn_in >= fw.t0) ? dtq0 : 0.0f;
@@ -1224,7 +1311,8 @@ int orc_raster_backward(const RScene *s, const CamIn *camin, const int64_t *offs
                         const int32_t *early_stop, const float *depth_q, int32_t Q, const float *log_t_in,
                         const float *g_color, const float *g_log_t, const float *g_ndist, const float *g_normal,
                         const float *g_qdepth, const float *g_contrib, float *gs_sph, float *gs_ns, float *gs_sites,
-                        float *gs_rgb, float *gs_h, int32_t nthreads) {
+                        float *gs_rgb, float *gs_h, int32_t flags, int32_t nthreads) {
+    const bool exact = (flags & 1) != 0;
     const Cam<float> cam = make_cam<float>(*camin);
     const int ntiles = ((cam.H + 7) / 8) * ((cam.W + 7) / 8);
     const float hw = (float)(cam.H * cam.W);
@@ -1337,8 +1425,10 @@ int orc_raster_backward(const RScene *s, const CamIn *camin, const int64_t *offs
                 else if (tf_id == n_adj) dLdts = dLdtf;
                 std::fill(red.begin(), red.end(), 0.0f);
                 V3<float> pdc, pdn;
-                float dtn_in;
-                rtex_bwd(o, d, tn0, c, n, r, act, tex.data(), K, tx, dLdcolor, dLdts, red.data(), pdc, pdn, dtn_in);
+                float dtn_in, dr_tex = 0.0f;
+                rtex_bwd(o, d, tn0, c, n, r, act, tex.data(), K, tx, dLdcolor, dLdts, red.data(), pdc, pdn, dtn_in,
+                         exact, dr_tex);
+                dLdr += dr_tex;
                 dLdc = add(dLdc, pdc);
                 dLdn = add(dLdn, pdn);
                 if (tn_id != n_adj) dtn_in += dLdtn;
