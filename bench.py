@@ -58,6 +58,9 @@ def main():
                     help="share of a step's capacity the oldest active request may take")
     ap.add_argument("--record-cap", type=int, default=128,
                     help="contributing segments per ray the forward records for the backward (0 = replay)")
+    ap.add_argument("--backward", choices=("grouped", "recorded", "replay"), default="grouped",
+                    help="walk backward: the record's segments sorted by cell (default), the record ray by ray, "
+                         "or checkpoint-and-replay without a record")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -124,6 +127,10 @@ def main():
                  texel_rgb=torch.zeros(args.cells, K, 3, device=dev),
                  texel_height=torch.zeros(args.cells, K, device=dev))
     g_coeffs = torch.zeros_like(coeffs)
+    if args.backward == "replay":
+        args.record_cap = 0
+    grouped = args.backward == "grouped"
+    workspace = rt.grouped_workspace(args.cells, 1, W, H, args.record_cap, dev) if grouped else None
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     times = {"fwd": [], "bwd": []}
 
@@ -141,7 +148,7 @@ def main():
         if record:
             ev[2].record()
         b = rt.launch_walk_backward(scene, cams_dev, W, H, f["color"], f["log_t"], gc, None, grads=grads,
-                                    record=f["record"])
+                                    record=f["record"], grouped=grouped, workspace=workspace)
         if record:
             ev[3].record()
         shading.sh_backward(coeffs, 3, sites, cams_dev, grads["texel_rgb"][None], grad_coeffs=g_coeffs)
@@ -175,7 +182,7 @@ def main():
     for k in GRAD_KEYS:
         grads[k].zero_()
     b = rt.launch_walk_backward(scene, cams_dev, W, H, f["color"], f["log_t"], gc, None, grads=grads,
-                                want_bytes=True, record=f["record"])
+                                want_bytes=True, record=f["record"], grouped=grouped, workspace=workspace)
     torch.cuda.synchronize()
     segs = None
     if f["record"] is not None:
@@ -242,6 +249,7 @@ def main():
             "mean_visits_per_ray": round(visits.mean().item(), 2),
             "walk_fwd_ms": round(fwd_ms, 4),
             "walk_bwd_ms": round(bwd_ms, 4),
+            "walk_backward": args.backward,
             "walk_fwd_alg_bytes": fwd_bytes,
             "walk_bwd_alg_bytes": bwd_bytes,
             "record_segments": segs,

@@ -303,7 +303,10 @@ __device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 
     const f3 q0 = ray_at(o, tq0, d);
     f3 dL_dq0 = mk(0, 0, 0);
     const float hdp_in = dL_dheight * fw.height;
-    if (dL_dheight != 0.0f) {
+    // A sink whose calls every lane makes together (kUniformHeight) runs
+    // this pass on every lane: lanes without a height gradient, and skipped
+    // texels, add zeros.
+    if (Sink::kUniformHeight || dL_dheight != 0.0f) {
         const float inv_ws0 = 1.0f / fw.wsum0;
 #pragma unroll
         for (int i = 0; i < (KT > 0 ? KT : 16); ++i) {
@@ -313,12 +316,20 @@ __device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 
             float w;
             if constexpr (KT > 0) w = fw.w0[i];
             else w = pf_expf(-kTemp * (dot(dv, dv) * inv_r2));
-            if (!exact && w / fw.wsum0 < 1e-3f) continue;
+            const bool keep = exact || !(w / fw.wsum0 < 1e-3f);
+            if constexpr (!Sink::kUniformHeight) {
+                if (!keep) continue;
+            }
             const float kw = k2 * w;
-            sink.height(i, (w * dL_dheight) * inv_ws0);
+            float hg = (w * dL_dheight) * inv_ws0;
             const float hdp = dL_dheight * (float)heights[i];
             const float coef = (hdp - hdp_in) * inv_ws0;
-            const f3 gs = (kw * coef) * dv;
+            f3 gs = (kw * coef) * dv;
+            if constexpr (Sink::kUniformHeight) {
+                hg = keep ? hg : 0.0f;
+                gs = keep ? gs : mk(0, 0, 0);
+            }
+            sink.height(i, hg);
             sink.site_div(i, gs);
             dL_dq0 = dL_dq0 + (-1.0f * gs);
             if (exact) dL_dr += coef * (kw * dot(dv, dv) / r);
@@ -415,7 +426,12 @@ struct SegRecord {
     int4 *rec;  // the record's first entry
     int64_t slot, S;
     int cap;
+    // optional, same indexing as rec: {transmittance before the segment,
+    // the ray's colour after it}, which makes a segment's gradients
+    // independent of the other segments of its ray
+    float4 *part = nullptr;
 };
+constexpr uint64_t kSegPartBytes = 16;
 constexpr uint64_t kSegRecordBytes = 16;
 
 __device__ __forceinline__ int4 pack_segment(int cell, float tn, float tf, int tn_id, int tf_id) {
@@ -499,10 +515,12 @@ __device__ __forceinline__ RayFwd walk_ray_fwd(const SceneDev<A> &s, const RGB &
         prim = next;
         const float dt = tf - tn;
         if (dt > 0.0f) {
+            int64_t rec_at = -1;
             if constexpr (REC) {
                 if (!(sh.tf <= 1e10f)) out.segs = out.segs > rec.cap ? out.segs : rec.cap;
                 if (out.segs < rec.cap) {
-                    rec.rec[rec_index(out.segs, rec.slot, rec.S)] = pack_segment(cell, tn_face, tf_face, tn_id, tf_id);
+                    rec_at = rec_index(out.segs, rec.slot, rec.S);
+                    rec.rec[rec_at] = pack_segment(cell, tn_face, tf_face, tn_id, tf_id);
                     out.bytes += kSegRecordBytes;
                 }
                 ++out.segs;
@@ -513,6 +531,12 @@ __device__ __forceinline__ RayFwd walk_ray_fwd(const SceneDev<A> &s, const RGB &
             out.rgb.y = __builtin_fmaf(tx.rgb.y * alpha, trans, out.rgb.y);
             out.rgb.z = __builtin_fmaf(tx.rgb.z * alpha, trans, out.rgb.z);
             out.log_t += dlt;
+            if constexpr (REC) {
+                if (rec.part && rec_at >= 0) {
+                    rec.part[rec_at] = make_float4(trans, out.rgb.x, out.rgb.y, out.rgb.z);
+                    out.bytes += kSegPartBytes;
+                }
+            }
             if (!depth_found) {
                 const float next_trans = pf_expf(out.log_t);
                 if (next_trans < depth_q) {
@@ -565,6 +589,7 @@ struct LanePair {
 // LanePair; site_div and height may be called from divergent code.
 template <bool PAIR> struct CellSink {
     static constexpr bool kUniform = PAIR;
+    static constexpr bool kUniformHeight = false;
     float *lds;
     float *gs, *gn, *gsite, *grgb, *gh;
     int K;
@@ -747,13 +772,12 @@ __device__ __forceinline__ uint64_t segment_grad_bytes(int K, int tn0_id, int tf
 // Gradients of one segment (rasterize.py:1371-1516, texture.py:74-186):
 //   dL/d(delta_s) = g.(C - C_{<=s}) + g_logT - (g.c_s) T_s exp(delta_s),
 // routed to the sphere, a power face or the texel plane.  The cell's own
-// gradients go to row (an LDS row of the wave's cache) or, row == nullptr,
-// to global memory; neighbour spheres get global atomics.
-template <typename A, typename F, int KT, typename RGB, bool PAIR = false>
-__device__ __forceinline__ void segment_grads(const SceneDev<A> &s, const RGB &colors, f3 o, f3 d, const Seg &q,
-                                              const TexFwdT<KT> &tx, f3 g, float gL, f3 Cfin, const GradDev &G,
-                                              bool exact, float *row,
-                                              LanePair pr = LanePair{true, false, false, false, false}) {
+// gradients go to sink (sphere, [n, sigma], texel sites, colours,
+// heights); neighbour spheres get global atomics.
+template <typename A, typename F, int KT, typename RGB, typename Sink>
+__device__ __forceinline__ void segment_grads_to(const SceneDev<A> &s, const RGB &colors, f3 o, f3 d, const Seg &q,
+                                                 const TexFwdT<KT> &tx, f3 g, float gL, f3 Cfin, const GradDev &G,
+                                                 bool exact, const Sink &sink) {
     const int K = KT > 0 ? KT : s.K;
     const int cell = q.cell, n_adj = q.n_adj;
     const f3 c = q.c;
@@ -771,14 +795,6 @@ __device__ __forceinline__ void segment_grads(const SceneDev<A> &s, const RGB &c
     else if (q.tf_id == n_adj) dL_dts = dL_dtf;
 
     const int64_t tbase = q.tbase;
-    const CellSink<PAIR> sink{row,
-                              G.spheres + 4 * (int64_t)cell,
-                              G.nsigmas + 4 * (int64_t)cell,
-                              G.sites + 3 * tbase,
-                              G.rgb + 3 * tbase,
-                              G.height + tbase,
-                              K,
-                              pr};
     const auto rgbs = colors.cell(cell, K);
     f3 dL_dc, dL_dn;
     float dL_dtn0, dL_dr = 0.0f;
@@ -812,6 +828,26 @@ __device__ __forceinline__ void segment_grads(const SceneDev<A> &s, const RGB &c
     }
     sink.sphere(dL_dc, dL_dr);
     sink.nsig(dL_dn, dL_dsigma);
+}
+
+// segment_grads_to with the cell's own gradients going to row (an LDS row
+// of the wave's cache) or, row == nullptr, to global memory.
+template <typename A, typename F, int KT, typename RGB, bool PAIR = false>
+__device__ __forceinline__ void segment_grads(const SceneDev<A> &s, const RGB &colors, f3 o, f3 d, const Seg &q,
+                                              const TexFwdT<KT> &tx, f3 g, float gL, f3 Cfin, const GradDev &G,
+                                              bool exact, float *row,
+                                              LanePair pr = LanePair{true, false, false, false, false}) {
+    const int K = KT > 0 ? KT : s.K;
+    const int64_t tbase = q.tbase;
+    const CellSink<PAIR> sink{row,
+                              G.spheres + 4 * (int64_t)q.cell,
+                              G.nsigmas + 4 * (int64_t)q.cell,
+                              G.sites + 3 * tbase,
+                              G.rgb + 3 * tbase,
+                              G.height + tbase,
+                              K,
+                              pr};
+    segment_grads_to<A, F, KT>(s, colors, o, d, q, tx, g, gL, Cfin, G, exact, sink);
 }
 
 // Backward by checkpoint and replay.  The forward's per-ray colour C and

@@ -9,55 +9,9 @@
 #include "pf_hip.h"
 #include "pf_launch.h"
 #include "pf_walk.h"
+#include "pf_walk_params.h"
 
 namespace pf {
-
-struct WalkParams {
-    pf_scene s;
-    const float *cams;  // (B, 12) pf_camera
-    int B, W, H, tiles_h, tiles_per_cam;
-    int64_t total_tiles;
-    float thr, depth_q;
-    int max_steps;
-    pf_walk_out out;
-    // backward only
-    const float *ck_color, *ck_log_t, *g_color, *g_log_t;
-    pf_grads grads;
-    int *status;
-    bool exact_texel;
-    uint64_t *alg_bytes;  // backward per-ray byte counter (may be null)
-    // record of contributing segments, indexed by lane slot (tile * 64 +
-    // lane): the forward writes it, the recorded backward reads it
-    int4 *rec_segs;
-    int *rec_counts;
-    int rec_cap;
-};
-
-struct TileLane {
-    int b, pix_i, pix_j;
-    bool valid;
-    int64_t ray, slot;
-};
-
-// block: the workgroup's index in the XCD-aware numbering of its tiles.
-__device__ __forceinline__ TileLane tile_lane(const WalkParams &p, int waves_per_block, int block) {
-    TileLane tl;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int64_t t = (int64_t)block * waves_per_block + wave;
-    tl.valid = t < p.total_tiles;
-    const int64_t tt = tl.valid ? t : 0;
-    tl.b = (int)(tt / p.tiles_per_cam);
-    const int tile = (int)(tt % p.tiles_per_cam);
-    const int tile_i = tile % p.tiles_h;
-    const int tile_j = tile / p.tiles_h;
-    tl.pix_i = tile_i * 8 + (lane & 7);
-    tl.pix_j = tile_j * 8 + (lane >> 3);
-    tl.valid = tl.valid && tl.pix_i < p.H && tl.pix_j < p.W;
-    tl.ray = ((int64_t)tl.b * p.H + tl.pix_i) * p.W + tl.pix_j;
-    tl.slot = tt * 64 + lane;
-    return tl;
-}
 
 template <typename A, typename F, int KT, bool REC>
 __global__ __launch_bounds__(256) void walk_fwd_kernel(WalkParams p) {
@@ -68,7 +22,7 @@ __global__ __launch_bounds__(256) void walk_fwd_kernel(WalkParams p) {
     const SceneDev<A> s = scene_dev<A>(p.s);
     const MemRGB<A> colors{reinterpret_cast<const A *>(p.s.texel_rgb) + (int64_t)tl.b * p.s.texel_rgb_cam_stride};
     int *seq = p.out.seq ? p.out.seq + tl.ray * p.out.seq_cap : nullptr;
-    const SegRecord rec{p.rec_segs, tl.slot, p.total_tiles * 64, p.rec_cap};
+    const SegRecord rec{p.rec_segs, tl.slot, p.total_tiles * 64, p.rec_cap, p.rec_part};
     const RayFwd r = walk_ray_fwd<A, F, KT, MemRGB<A>, REC>(s, colors, cam.eye, d, cam.start, p.thr, p.depth_q,
                                                              p.max_steps, seq, p.out.seq_cap, rec);
     if constexpr (REC) p.rec_counts[tl.slot] = r.segs;
@@ -100,6 +54,7 @@ constexpr int kRowsPerWave = kCacheSlots * WaveCache::kPitch;
 template <typename A, typename F, int KT>
 __device__ __forceinline__ void replay_tile(const WalkParams &p, const TileLane &tl, bool only_over, float *rows) {
     constexpr bool kWave = KT == 8;
+    if (only_over && p.rec_header && p.rec_header[7] != 1) return;  // a record of another walk
     if (!kWave && !tl.valid) return;
     const bool mine = tl.valid && (!only_over || p.rec_counts[tl.slot] > p.rec_cap);
     if (!kWave && !mine) return;
@@ -142,6 +97,7 @@ __device__ __forceinline__ void replay_tile(const WalkParams &p, const TileLane 
 template <typename A, typename F, int KT>
 __device__ __forceinline__ void record_tile(const WalkParams &p, const TileLane &tl, float *rows) {
     constexpr bool kWave = KT == 8;
+    if (p.rec_header && p.rec_header[7] != 1) return;  // a record of another walk
     if (!kWave && !tl.valid) return;
     const int count = tl.valid ? p.rec_counts[tl.slot] : 0;
     const bool mine = tl.valid && count <= p.rec_cap;
@@ -290,9 +246,63 @@ __global__ void start_write_kernel(const unsigned long long *ws, float *cams, in
     if (b < B) reinterpret_cast<int *>(cams)[12 * b + 9] = (int)(uint32_t)(ws[b] & 0xffffffffull);
 }
 
+// Record stamp: the scene's cell and edge counts, the cap, the tile count
+// and a hash of the camera words, written by the recording forward and
+// compared by each backward that reads the record.  The hash sums a mix of
+// each (index, word) pair (the 32-bit "lowbias32" integer hash of
+// C. Wellons), so one block computes it in parallel.
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+constexpr int kStampMagic = 0x50465752;  // "RWFP"
+
+__global__ __launch_bounds__(256) void record_stamp_kernel(const int *cams, int64_t words, int64_t N, int64_t E,
+                                                           int cap, int64_t tiles, int *header, int check,
+                                                           int *status) {
+    uint32_t h = 0;
+    for (int64_t i = threadIdx.x; i < words; i += 256) h += mix32((uint32_t)cams[i] ^ mix32((uint32_t)i * 0x9E3779B9u));
+    for (int off = 32; off > 0; off >>= 1) h += (uint32_t)__shfl_down((int)h, off, 64);
+    __shared__ uint32_t part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = h;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    h = part[0] + part[1] + part[2] + part[3];
+    const int want[7] = {kStampMagic, (int)N, (int)(E & 0x7fffffff), (int)(E >> 31), cap, (int)tiles, (int)h};
+    if (!check) {
+        for (int k = 0; k < 7; ++k) header[k] = want[k];
+        header[7] = 1;
+        return;
+    }
+    bool ok = true;
+    for (int k = 0; k < 7; ++k) ok = ok && header[k] == want[k];
+    header[7] = ok ? 1 : 0;
+    if (!ok && status) atomicOr(status, PF_STATUS_BAD_RECORD);
+}
+
 }  // namespace pf
 
 using namespace pf;
+
+// Stamps (check == 0) or checks (check == 1) a record's header; no-op
+// without one.
+static int record_stamp(const pf_walk_record *r, const pf_scene *s, const pf_camera *cams, int32_t num_cams,
+                        int64_t total_tiles, int check, int32_t *status, hipStream_t st) {
+    if (!r->header) return PF_OK;
+    hipLaunchKernelGGL(record_stamp_kernel, dim3(1), dim3(256), 0, st, reinterpret_cast<const int *>(cams),
+                       (int64_t)num_cams * 12, s->num_cells, s->num_edges, r->cap, total_tiles, r->header, check,
+                       status);
+    return hipGetLastError() == hipSuccess ? PF_OK : PF_ERR_HIP;
+}
+
+int pf_walk_check_stamp(const pf_walk_record *record, const pf_scene *scene, const pf_camera *cams,
+                        int32_t num_cams, int64_t total_tiles, int32_t *status, hipStream_t st) {
+    return record_stamp(record, scene, cams, num_cams, total_tiles, 1, status, st);
+}
 
 enum WalkMode { kFwd, kFwdRecord, kBwd, kBwdRecord };
 
@@ -380,6 +390,9 @@ extern "C" int pf_walk_forward(const pf_scene *scene, const pf_camera *cams, int
         p.rec_segs = reinterpret_cast<int4 *>(r->segments);
         p.rec_counts = r->counts;
         p.rec_cap = r->cap;
+        p.rec_part = reinterpret_cast<float4 *>(r->partials);
+        rc = record_stamp(r, scene, cams, num_cams, p.total_tiles, 0, nullptr, (hipStream_t)stream);
+        if (rc) return rc;
         return dispatch_walk(p, kFwdRecord, (hipStream_t)stream);
     }
     return dispatch_walk(p, kFwd, (hipStream_t)stream);
@@ -440,6 +453,9 @@ extern "C" int pf_walk_backward_recorded(const pf_scene *scene, const pf_camera 
     p.rec_segs = reinterpret_cast<int4 *>(record->segments);
     p.rec_counts = record->counts;
     p.rec_cap = record->cap;
+    p.rec_header = record->header;
+    rc = record_stamp(record, scene, cams, num_cams, p.total_tiles, 1, status, (hipStream_t)stream);
+    if (rc) return rc;
     return dispatch_walk(p, kBwdRecord, (hipStream_t)stream);
 }
 

@@ -64,6 +64,8 @@ class PfWalkRecord(ctypes.Structure):
         ("counts", c_void_p),
         ("cap", c_int32),
         ("reserved", c_int32),
+        ("partials", c_void_p),
+        ("header", c_void_p),
     ]
 
 
@@ -147,6 +149,13 @@ _SIGS = {
         [ctypes.POINTER(PfScene), c_void_p, c_int32, c_int32, c_int32, c_float, c_int32, c_void_p, c_void_p,
          c_void_p, c_void_p, ctypes.POINTER(PfWalkRecord), ctypes.POINTER(PfGrads), c_int32, c_void_p, c_void_p,
          c_void_p],
+    ),
+    "pf_walk_grouped_workspace_bytes": (c_int64, [c_int64, c_int32, c_int32, c_int32, c_int32]),
+    "pf_walk_backward_grouped": (
+        c_int32,
+        [ctypes.POINTER(PfScene), c_void_p, c_int32, c_int32, c_int32, c_float, c_int32, c_void_p, c_void_p,
+         c_void_p, c_void_p, ctypes.POINTER(PfWalkRecord), ctypes.POINTER(PfGrads), c_int32, c_void_p, c_int64,
+         c_void_p, c_void_p, c_void_p],
     ),
     "pf_walk_faces": (
         c_int32,

@@ -21,7 +21,7 @@ DEFAULT_MAX_STEPS = 1 << 16
 # contributing segments per ray the differentiable forward records for its
 # backward, and the most device memory one record may take
 DEFAULT_RECORD_CAP = 128
-DEFAULT_RECORD_BUDGET = 8 << 30
+DEFAULT_RECORD_BUDGET = 12 << 30
 
 
 def _contig(t, dtype=None):
@@ -52,8 +52,14 @@ class WalkScene:
         if self.texel_sites.dim() != 3 or self.texel_sites.shape[0] != n:
             raise ValueError("texel_sites must be (N, K, 3)")
         self.K = self.texel_sites.shape[1]
-        if tuple(self.texel_rgb.shape[-3:]) != (n, self.K, 3) or tuple(self.texel_height.shape) != (n, self.K):
-            raise ValueError("texel_rgb must be (N, K, 3) and texel_height (N, K)")
+        # texel_rgb (N, K, 3), or (B, N, K, 3): one set of colours per camera
+        # (the view-dependent colour pre-pass of each camera)
+        self.rgb_cams = 0
+        if self.texel_rgb.dim() == 4:
+            self.rgb_cams = self.texel_rgb.shape[0]
+        if tuple(self.texel_rgb.shape[-3:]) != (n, self.K, 3) or self.texel_rgb.dim() not in (3, 4) or \
+                tuple(self.texel_height.shape) != (n, self.K):
+            raise ValueError("texel_rgb must be (N, K, 3) or (B, N, K, 3) and texel_height (N, K)")
         if adjacency.dtype not in (torch.int32, torch.int64) or adjacency_offsets.dtype not in (torch.int32,
                                                                                                 torch.int64):
             raise ValueError("adjacency / adjacency_offsets must be integer tensors")
@@ -87,7 +93,17 @@ class WalkScene:
         s.num_texel_sites = self.K
         s.attr_dtype = _lib.dtype_code(attr_dtype)
         s.face_dtype = face_code
+        s.texel_rgb_cam_stride = n * self.K * 3 if self.rgb_cams else 0
         self.c = s
+
+
+
+
+def _check_rgb_cams(scene, B):
+    """Per-camera texel colours must cover exactly the launch's cameras."""
+    n = getattr(scene, "rgb_cams", 0)
+    if n and n != B:
+        raise ValueError(f"texel_rgb holds colours for {n} cameras, the launch has {B}")
 
 
 def start_cells_device(points, radii, cams_dev):
@@ -138,24 +154,31 @@ def record_slots(num_cams, width, height):
     return num_cams * ((height + 7) // 8) * ((width + 7) // 8) * 64
 
 
-def record_bytes(num_cams, width, height, cap):
-    return 16 * cap * record_slots(num_cams, width, height) + 4 * record_slots(num_cams, width, height)
+def record_bytes(num_cams, width, height, cap, partials=True):
+    """Device bytes of a record: 16 per segment (32 with partials) and 4 per
+    slot, plus its 32-byte stamp."""
+    S = record_slots(num_cams, width, height)
+    return (32 if partials else 16) * cap * S + 4 * S + 32
 
 
 RECORD_CHUNK = 8  # segments of one ray per 128-byte line; caps are multiples
 
 
-def new_record(num_cams, width, height, cap, device):
+def new_record(num_cams, width, height, cap, device, partials=True):
     """Device buffers of a walk record holding up to cap contributing
-    segments per ray (16 bytes each; cap rounded up to a multiple of 8) and
-    every ray's count.  Segment k of slot s is row ((k // 8) * S + s) * 8 +
-    k % 8 of "segments" viewed as (cap * S, 4)."""
+    segments per ray (16 bytes each; cap rounded up to a multiple of 8),
+    every ray's count, the stamp the forward writes and, with partials, each
+    segment's transmittance before it and the ray's colour after it (16
+    bytes more, which the grouped backward needs).  Segment k of slot s is
+    row ((k // 8) * S + s) * 8 + k % 8 of "segments" viewed as (cap * S, 4)."""
     if cap <= 0:
         raise ValueError("record cap must be positive")
     cap = -(-int(cap) // RECORD_CHUNK) * RECORD_CHUNK
     S = record_slots(num_cams, width, height)
     return dict(segments=torch.empty(cap * S, 4, dtype=torch.int32, device=device),
-                counts=torch.empty(S, dtype=torch.int32, device=device), cap=cap)
+                counts=torch.empty(S, dtype=torch.int32, device=device), cap=cap,
+                partials=torch.empty(cap * S, 4, dtype=torch.float32, device=device) if partials else None,
+                header=torch.zeros(8, dtype=torch.int32, device=device))
 
 
 def record_segment(record, k, slot):
@@ -178,7 +201,19 @@ def _record_struct(record):
     r.segments = _lib.ptr(record["segments"])
     r.counts = _lib.ptr(record["counts"])
     r.cap = record["cap"]
+    r.partials = _lib.ptr(record.get("partials"))
+    r.header = _lib.ptr(record.get("header"))
     return r
+
+
+def grouped_workspace(num_cells, num_cams, width, height, cap, device):
+    """Device workspace of the grouped backward (bucket offsets, cursors,
+    the cell-sorted segment ids)."""
+    lib = _lib.load()
+    n = lib.pf_walk_grouped_workspace_bytes(int(num_cells), int(num_cams), int(width), int(height), int(cap))
+    if n <= 0:
+        raise ValueError("sizes out of range for the grouped backward")
+    return torch.empty(n, dtype=torch.uint8, device=device)
 
 
 def _expect(t, shape, name, dtype=torch.float32, device=None):
@@ -202,6 +237,7 @@ def launch_walk_forward(scene, cams_dev, width, height, transmittance_threshold=
     B = cams_dev.shape[0]
     dev = cams_dev.device
     _expect(cams_dev, (B, 12), "cams_dev", dtype=torch.int32)
+    _check_rgb_cams(scene, B)
     color = torch.empty(B, height, width, 3, dtype=torch.float32, device=dev)
     log_t = torch.empty(B, height, width, dtype=torch.float32, device=dev) if want_log_t else None
     depth = torch.empty(B, height, width, dtype=torch.float32, device=dev) if want_depth else None
@@ -238,26 +274,37 @@ def launch_walk_forward(scene, cams_dev, width, height, transmittance_threshold=
 
 def launch_walk_backward(scene, cams_dev, width, height, color, log_t, grad_color, grad_log_t,
                          transmittance_threshold=1e-3, max_steps=DEFAULT_MAX_STEPS, grads=None, exact_texel=False,
-                         texel_rgb_cam_stride=0, want_bytes=False, record=None):
+                         texel_rgb_cam_stride=0, want_bytes=False, record=None, grouped=None, workspace=None):
     """Gradients of the walk.  record: the forward's record (its "record"
-    output), read instead of replaying every ray; None replays."""
+    output), read instead of replaying every ray; None replays.  grouped:
+    differentiate the record's segments sorted by cell
+    (pf_walk_backward_grouped); None: whenever the record has partials and
+    K = 8.  workspace: its device workspace (allocated if None)."""
     lib = _lib.load()
     n, K = scene.num_cells, scene.K
     dev = cams_dev.device
     B = cams_dev.shape[0]
     _expect(cams_dev, (B, 12), "cams_dev", dtype=torch.int32)
+    _check_rgb_cams(scene, B)
     _expect(color, (B, height, width, 3), "color", device=dev)
     _expect(log_t, (B, height, width), "log_t", device=dev)
     _expect(grad_color, (B, height, width, 3), "grad_color", device=dev)
     _expect(grad_log_t, (B, height, width), "grad_log_t", device=dev)
     if grads is None:
+        per_cam = getattr(scene, "rgb_cams", 0) > 0
         grads = dict(
             spheres=torch.zeros(n, 4, dtype=torch.float32, device=dev),
             nsigmas=torch.zeros(n, 4, dtype=torch.float32, device=dev),
             texel_sites=torch.zeros(n, K, 3, dtype=torch.float32, device=dev),
-            texel_rgb=torch.zeros(n, K, 3, dtype=torch.float32, device=dev),
+            texel_rgb=torch.zeros((B, n, K, 3) if per_cam else (n, K, 3), dtype=torch.float32, device=dev),
             texel_height=torch.zeros(n, K, dtype=torch.float32, device=dev),
         )
+        if per_cam:
+            texel_rgb_cam_stride = n * K * 3
+    elif getattr(scene, "rgb_cams", 0) and not texel_rgb_cam_stride:
+        if tuple(grads["texel_rgb"].shape) != (B, n, K, 3):
+            raise ValueError("per-camera texel colours need per-camera gradients: grads['texel_rgb'] (B, N, K, 3)")
+        texel_rgb_cam_stride = n * K * 3
     for k, shape in (("spheres", (n, 4)), ("nsigmas", (n, 4)), ("texel_sites", (n, K, 3)),
                      ("texel_rgb", (n, K, 3)), ("texel_height", (n, K))):
         t = grads[k]
@@ -275,7 +322,24 @@ def launch_walk_backward(scene, cams_dev, width, height, color, log_t, grad_colo
     status = torch.zeros(1, dtype=torch.int32, device=dev)
     alg_bytes = torch.zeros(color.shape[:-1], dtype=torch.int64, device=dev) if want_bytes else None
     flags = _lib.PF_BWD_EXACT_TEXEL if exact_texel else 0
-    if record is not None:
+    if record is not None and grouped is None:
+        grouped = record.get("partials") is not None and K == 8
+    if record is not None and grouped:
+        if record["counts"].numel() != record_slots(B, width, height):
+            raise ValueError("the record belongs to a walk of another size")
+        if record.get("partials") is None or K != 8:
+            raise ValueError("the grouped backward needs a record with partials and K = 8 texels")
+        if workspace is None:
+            workspace = grouped_workspace(n, B, width, height, record["cap"], dev)
+        rec = _record_struct(record)
+        rc = lib.pf_walk_backward_grouped(ctypes.byref(scene.c), _lib.ptr(cams_dev), B, width, height,
+                                          float(transmittance_threshold), int(max_steps), _lib.ptr(color),
+                                          _lib.ptr(log_t), _lib.ptr(grad_color), _lib.ptr(grad_log_t),
+                                          ctypes.byref(rec), ctypes.byref(g), flags, _lib.ptr(workspace),
+                                          workspace.numel(), _lib.ptr(status), _lib.ptr(alg_bytes),
+                                          _lib.stream_ptr())
+        _lib.check(rc, "pf_walk_backward_grouped")
+    elif record is not None:
         if record["counts"].numel() != record_slots(cams_dev.shape[0], width, height):
             raise ValueError("the record belongs to a walk of another size")
         rec = _record_struct(record)
@@ -365,11 +429,13 @@ class RayTracer:
 
     def record_cap_for(self, num_cams, width, height):
         """Segments per ray the differentiable forward records: record_cap,
-        lowered to fit record_budget bytes; 0 (replay) below 8."""
+        lowered to fit record_budget bytes (36 per segment slot: the
+        segment, its partials and its sorted id in the grouped backward's
+        workspace); 0 (replay) below 8."""
         if self.record_cap <= 0:
             return 0
-        fit = (self.record_budget - 4 * record_slots(num_cams, width, height)) // \
-            (16 * record_slots(num_cams, width, height))
+        fit = (self.record_budget - 8 * record_slots(num_cams, width, height)) // \
+            (36 * record_slots(num_cams, width, height))
         cap = min(self.record_cap, int(fit)) // RECORD_CHUNK * RECORD_CHUNK
         return cap if cap >= 8 else 0
 

@@ -37,8 +37,10 @@ extern "C" {
                                      forever (raytrace.py:163 never ends on
                                      such a ray); the ray stops there      */
 #define PF_STATUS_BAD_RECORD  8   /* a walk record named a cell outside the
-                                     scene (a record of another walk); the
-                                     segment was skipped                    */
+                                     scene, or its stamp names another
+                                     scene, size or set of cameras (a
+                                     record of another walk); the segment,
+                                     or the whole record, was skipped       */
 
 /* ---- dtypes ------------------------------------------------------------ */
 #define PF_F32 0
@@ -86,8 +88,9 @@ typedef struct pf_scene {
 
 /* Record of each ray's contributing segments (segments with positive
  * length after the face and texel bounds), written by pf_walk_forward for
- * pf_walk_backward_recorded to read instead of replaying the walk.  No
- * reference counterpart: the reference has no walk backward.
+ * pf_walk_backward_recorded / pf_walk_backward_grouped to read instead of
+ * replaying the walk.  No reference counterpart: the reference has no walk
+ * backward.
  * Slots: one per lane of the walk's 8x8 tiles, slot = tile * 64 + lane,
  * tiles numbered camera-major then column-major over ceil(H/8) x ceil(W/8);
  * pf_walk_record_slots gives their number S.
@@ -99,12 +102,23 @@ typedef struct pf_scene {
  *           with more than cap keep only their first cap, and the backward
  *           replays those rays instead.
  * cap:      a positive multiple of 8.
- * 16 * cap * S bytes of device memory (2.1 GB at cap 64, 1920x1080).      */
+ * partials: NULL, or cap * S float4 {T before the segment, the ray's
+ *           colour after it}, indexed as segments; required by
+ *           pf_walk_backward_grouped.
+ * header:   NULL, or 8 int32 that pf_walk_forward stamps with the scene's
+ *           cell and edge counts, the cap, the slot count and a hash of the
+ *           camera words; a backward given a record whose stamp differs
+ *           from its own arguments reports PF_STATUS_BAD_RECORD and adds no
+ *           gradient.
+ * 16 * cap * S bytes of device memory for segments (2.1 GB at cap 64,
+ * 1920x1080), as much again for partials.                                  */
 typedef struct pf_walk_record {
     void    *segments;
     int32_t *counts;
     int32_t  cap;
     int32_t  reserved;
+    void    *partials;
+    int32_t *header;
 } pf_walk_record;
 
 /* Per-ray outputs of the walk.  Any pointer may be NULL except color.
@@ -203,6 +217,40 @@ int pf_walk_backward_recorded(const pf_scene *scene, const pf_camera *cams,
                               const pf_grads *grads, int32_t flags,
                               int32_t *status, uint64_t *alg_bytes,
                               void *stream);
+
+/* Bytes of device workspace pf_walk_backward_grouped needs for a record
+ * of cap segments per ray over num_cams cameras of width x height and a
+ * scene of num_cells cells; 0 if the sizes are out of range.              */
+int64_t pf_walk_grouped_workspace_bytes(int64_t num_cells, int32_t num_cams,
+                                        int32_t width, int32_t height,
+                                        int32_t cap);
+
+/* Power-cell walk, backward from the forward's record, grouped by cell.
+ * No reference counterpart, as pf_walk_backward; it replaces the atomic
+ * accumulation of the reference's raster backward (rasterize.py:1507-1516,
+ * texture.py:125-171) for the walk.  The record must carry partials (the
+ * transmittance before and the colour after each segment), which make every
+ * segment's gradients independent of the rest of its ray.  The recorded
+ * segments are counted and sorted by cell on the device (count, scan,
+ * scatter), then differentiated in cell order with every lane busy; the
+ * terms of equal cells are summed across lanes and added to grads once per
+ * run of a cell.  Rays the record could not hold whole are replayed as
+ * pf_walk_backward does, in the same launch.  K = 8 texels (other K: use
+ * pf_walk_backward_recorded).  Results equal pf_walk_backward_recorded's up
+ * to the order of float additions.
+ * workspace: device memory of pf_walk_grouped_workspace_bytes bytes
+ * (contents need not be initialised).  Other arguments as
+ * pf_walk_backward_recorded; alg_bytes (B,H,W) per ray, or NULL.           */
+int pf_walk_backward_grouped(const pf_scene *scene, const pf_camera *cams,
+                             int32_t num_cams, int32_t width, int32_t height,
+                             float transmittance_threshold, int32_t max_steps,
+                             const float *color, const float *log_t,
+                             const float *grad_color, const float *grad_log_t,
+                             const pf_walk_record *record,
+                             const pf_grads *grads, int32_t flags,
+                             void *workspace, int64_t workspace_bytes,
+                             int32_t *status, uint64_t *alg_bytes,
+                             void *stream);
 
 /* Walk face precompute (benchmark.py:295-302): for every CSR edge i->j,
  * faces[e] = half(p_j - p_i, pm_j - pm_i) with pm = 0.5(|p|^2 - r^2).

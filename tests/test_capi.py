@@ -41,12 +41,12 @@ def test_struct_layouts_match_header():
 
     # the same sizes are static_asserts in csrc/capi.hip, so the library
     # and this binding agree or the library does not build
-    sizes = dict(PfCamera=48, PfScene=104, PfWalkOut=80, PfWalkRecord=24, PfGrads=48, PfServeScene=120,
+    sizes = dict(PfCamera=48, PfScene=104, PfWalkOut=80, PfWalkRecord=40, PfGrads=48, PfServeScene=120,
                  PfServeRequest=160, PfCechGrid=32)
     for name, size in sizes.items():
         assert ctypes.sizeof(getattr(_lib, name)) == size, name
     src = open(os.path.join(ROOT, "dynamic-powerfoam_amd", "csrc", "capi.hip")).read()
-    for cname, size in (("pf_camera", 48), ("pf_scene", 104), ("pf_walk_out", 80), ("pf_walk_record", 24),
+    for cname, size in (("pf_camera", 48), ("pf_scene", 104), ("pf_walk_out", 80), ("pf_walk_record", 40),
                         ("pf_grads", 48), ("pf_serve_scene", 120), ("pf_serve_request", 160),
                         ("pf_cech_grid", 32)):
         assert f"sizeof({cname}) == {size}" in src, cname
@@ -97,6 +97,23 @@ def test_walk_arguments_rejected_before_any_launch():
                                 ctypes.byref(grads), 0, None, None, None) == _lib.PF_ERR_ARG
     assert lib.pf_walk_record_slots(2, 9, 17) == 2 * 3 * 2 * 64
     assert lib.pf_walk_record_slots(0, 9, 17) == 0
+    # the grouped backward: partials required, K = 8, a large enough workspace
+    ws = lib.pf_walk_grouped_workspace_bytes(10, 1, 8, 8, 16)
+    assert ws >= 4 * 64 * 16 and lib.pf_walk_grouped_workspace_bytes(10, 1, 8, 8, 0) == 0
+    # entry ids are int32: 2^31 record entries and more are refused
+    assert lib.pf_walk_grouped_workspace_bytes(10, 1, 8192, 8192, 2048) == 0
+    grouped = lambda r, w=fake, nb=ws, s=scene: lib.pf_walk_backward_grouped(
+        ctypes.byref(s), fake, 1, 8, 8, 1e-4, 64, fake, fake, fake, None, ctypes.byref(r), ctypes.byref(grads), 0,
+        w, nb, None, None, None)
+    no_part = _lib.PfWalkRecord(segments=fake, counts=fake, cap=16)
+    assert grouped(no_part) == _lib.PF_ERR_ARG
+    rec16 = _lib.PfWalkRecord(segments=fake, counts=fake, cap=16, partials=fake)
+    assert grouped(rec16, nb=ws - 1) == _lib.PF_ERR_OVERFLOW
+    assert grouped(rec16, w=None) == _lib.PF_ERR_OVERFLOW
+    k4 = _lib.PfScene.from_buffer_copy(scene)
+    k4.num_texel_sites = 4
+    assert grouped(rec16, s=k4) == _lib.PF_ERR_ARG
+    assert grouped(_lib.PfWalkRecord(segments=fake, counts=fake, cap=12, partials=fake)) == _lib.PF_ERR_ARG
 
 
 def test_raster_and_serving_sizes_rejected_before_any_launch():

@@ -1,5 +1,7 @@
 """The recorded backward: the forward writes each ray's contributing
-segments and the backward differentiates them without replaying the walk.
+segments and the backward differentiates them without replaying the walk,
+ray by ray (pf_walk_backward_recorded) or sorted by cell
+(pf_walk_backward_grouped).
 
 Bar: the recording forward's outputs bit-equal to the plain forward's; the
 record's cells an in-order subsequence of the visited sequence; gradients
@@ -41,12 +43,21 @@ def _run(sc, cams, cap, exact=False):
     gc = 2.0 * (fwd["color"] - target)
     gl = -0.3 * torch.exp(fwd["log_t"])
     rec = rt.launch_walk_backward(gs, cams_dev, W, H, fwd["color"], fwd["log_t"], gc, gl, exact_texel=exact,
-                                  record=fwd["record"])
+                                  record=fwd["record"], grouped=False)
     rep = rt.launch_walk_backward(gs, cams_dev, W, H, fwd["color"], fwd["log_t"], gc, gl, exact_texel=exact)
+    grp = None
+    if gs.K == 8:
+        grp = rt.launch_walk_backward(gs, cams_dev, W, H, fwd["color"], fwd["log_t"], gc, gl, exact_texel=exact,
+                                      record=fwd["record"], grouped=True)
     torch.cuda.synchronize()
     for key in ("color", "log_t", "depth", "visits", "seq_hash", "seq"):
         assert torch.equal(plain[key], fwd[key]), key
     assert int(rec["status"].item()) == 0
+    if grp is not None:
+        # the grouped backward: the same segments summed in cell order
+        assert int(grp["status"].item()) == 0
+        for key in KEYS:
+            assert _rel(grp[key].cpu().numpy(), rep[key].cpu().numpy()) < 1e-5, key
     return words, gs, faces, fwd, gc, gl, rec, rep
 
 
@@ -169,10 +180,77 @@ def test_record_of_another_walk_is_reported_not_read(scene_textured):
     slot = int(torch.nonzero(r["counts"] > 0)[0, 0].item())
     r["segments"][rt.record_segment(r, 0, slot), 0] = sc["points"].shape[0]  # one past the last cell
     gc = torch.ones_like(fwd["color"])
-    g = rt.launch_walk_backward(gs, cams_dev, 16, 16, fwd["color"], fwd["log_t"], gc, None, record=r)
-    torch.cuda.synchronize()
-    assert int(g["status"].item()) & _lib.PF_STATUS_BAD_RECORD
-    assert all(bool(g[k].isfinite().all()) for k in KEYS)
+    for grouped in (False, True):
+        g = rt.launch_walk_backward(gs, cams_dev, 16, 16, fwd["color"], fwd["log_t"], gc, None, record=r,
+                                    grouped=grouped)
+        torch.cuda.synchronize()
+        assert int(g["status"].item()) & _lib.PF_STATUS_BAD_RECORD
+        assert all(bool(g[k].isfinite().all()) for k in KEYS)
+
+
+@pytest.mark.parametrize("grouped", [False, True])
+def test_record_stamp_of_other_cameras_refused(scene_textured, grouped):
+    """A record written for other cameras (in-range cells, a stale record)
+    fails the stamp check: PF_STATUS_BAD_RECORD and no gradient at all;
+    the record's own cameras pass it."""
+    from powerfoam import _lib
+
+    sc = scene_textured
+    cams = [workloads.front_camera(16, 16, hfov_deg=40.0)]
+    words = _cams(sc, cams)
+    gs, _ = _gpu_scene(sc)
+    cams_dev = torch.from_numpy(words).to(DEV)
+    fwd = rt.launch_walk_forward(gs, cams_dev, 16, 16, record_cap=64)
+    other = words.copy()
+    other[0, 0] = np.float32(other[0, :1].view(np.float32)[0] + 0.01).view(np.int32)  # the eye moved
+    gc = torch.ones_like(fwd["color"])
+    for w, bad in ((other, True), (words, False)):
+        g = rt.launch_walk_backward(gs, torch.from_numpy(w).to(DEV), 16, 16, fwd["color"], fwd["log_t"], gc, None,
+                                    record=fwd["record"], grouped=grouped)
+        torch.cuda.synchronize()
+        st = int(g["status"].item())
+        assert bool(st & _lib.PF_STATUS_BAD_RECORD) == bad, st
+        total = sum(float(g[k].abs().sum().item()) for k in KEYS)
+        assert (total == 0.0) == bad, total
+
+
+def test_grouped_backward_per_camera_colours(scene_textured):
+    """Three cameras, each with its own texel colours (the colour pre-pass
+    per camera): the grouped backward sums runs per (camera, cell), so each
+    camera's colour gradients equal that camera's alone and the shared
+    arrays equal the sum over the cameras."""
+    sc = scene_textured
+    cams = workloads.sphere_cameras(np.random.default_rng(3), 3, 24, 24, radius=2.5)
+    words = _cams(sc, cams)
+    rng = np.random.default_rng(4)
+    rgb = rng.uniform(0, 1, (3,) + sc["texel_rgb"].shape).astype(np.float32)
+    gs1, faces = _gpu_scene(sc)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+    gsb = rt.WalkScene(t(sc["points"]), t(sc["radii"]), t(sc["density"]), t(sc["normals"]), t(sc["texel_sites"]),
+                       t(rgb), t(sc["texel_height"]), t(sc["adjacency"]), t(sc["offsets"]), gs1.faces)
+    cams_dev = t(words)
+    fwd = rt.launch_walk_forward(gsb, cams_dev, 24, 24, record_cap=64)
+    gc = 2.0 * (fwd["color"] - 0.4)
+    grp = rt.launch_walk_backward(gsb, cams_dev, 24, 24, fwd["color"], fwd["log_t"], gc, None, record=fwd["record"],
+                                  grouped=True)
+    rec = rt.launch_walk_backward(gsb, cams_dev, 24, 24, fwd["color"], fwd["log_t"], gc, None, record=fwd["record"],
+                                  grouped=False)
+    assert tuple(grp["texel_rgb"].shape) == (3,) + sc["texel_rgb"].shape
+    sums = {k: 0.0 for k in KEYS if k != "texel_rgb"}
+    for b in range(3):
+        gsk = rt.WalkScene(t(sc["points"]), t(sc["radii"]), t(sc["density"]), t(sc["normals"]),
+                           t(sc["texel_sites"]), t(rgb[b]), t(sc["texel_height"]), t(sc["adjacency"]),
+                           t(sc["offsets"]), gs1.faces)
+        one = rt.launch_walk_backward(gsk, cams_dev[b:b + 1].contiguous(), 24, 24, fwd["color"][b:b + 1].contiguous(),
+                                      fwd["log_t"][b:b + 1].contiguous(), gc[b:b + 1].contiguous(), None)
+        torch.cuda.synchronize()
+        assert _rel(grp["texel_rgb"][b].cpu().numpy(), one["texel_rgb"].cpu().numpy()) < 1e-5, b
+        for k in sums:
+            sums[k] = sums[k] + one[k].cpu().numpy()
+    for k in sums:
+        assert _rel(grp[k].cpu().numpy(), sums[k]) < 1e-5, k
+    for k in KEYS:
+        assert _rel(grp[k].cpu().numpy(), rec[k].cpu().numpy()) < 1e-5, k
 
 
 @pytest.mark.parametrize("attr,f16,packed", [(torch.float16, True, True), (torch.float32, False, False),
@@ -191,8 +269,12 @@ def test_recorded_backward_dtypes(scene_textured, attr, f16, packed):
     for key in ("color", "log_t", "depth", "visits", "seq_hash"):
         assert torch.equal(plain[key], fwd[key]), key
     gc = 2.0 * (fwd["color"] - 0.5)
-    rec = rt.launch_walk_backward(gs, cams_dev, 40, 32, fwd["color"], fwd["log_t"], gc, None, record=fwd["record"])
+    rec = rt.launch_walk_backward(gs, cams_dev, 40, 32, fwd["color"], fwd["log_t"], gc, None, record=fwd["record"],
+                                  grouped=False)
+    grp = rt.launch_walk_backward(gs, cams_dev, 40, 32, fwd["color"], fwd["log_t"], gc, None, record=fwd["record"],
+                                  grouped=True)
     rep = rt.launch_walk_backward(gs, cams_dev, 40, 32, fwd["color"], fwd["log_t"], gc, None)
     torch.cuda.synchronize()
     for key in KEYS:
         assert _rel(rec[key].cpu().numpy(), rep[key].cpu().numpy()) < 1e-5, key
+        assert _rel(grp[key].cpu().numpy(), rep[key].cpu().numpy()) < 1e-5, key

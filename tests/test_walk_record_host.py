@@ -90,6 +90,16 @@ def test_walk_launch_buffers_are_checked_before_any_launch():
         bwd(grads=good, texel_rgb_cam_stride=240)  # two cameras' rows need 480 floats
     with pytest.raises(ValueError, match="cams_dev"):
         rt.launch_walk_forward(scene, torch.zeros(B, 12), W, H)
+    # per-camera texel colours must match the launch's cameras, and their
+    # gradients must be per camera too
+    per_cam = types.SimpleNamespace(num_cells=10, K=8, rgb_cams=3)
+    with pytest.raises(ValueError, match="3 cameras"):
+        rt.launch_walk_forward(per_cam, cams, W, H)
+    with pytest.raises(ValueError, match="3 cameras"):
+        rt.launch_walk_backward(per_cam, cams, W, H, color, log_t, color, None)
+    per_cam2 = types.SimpleNamespace(num_cells=10, K=8, rgb_cams=2)
+    with pytest.raises(ValueError, match="per-camera"):
+        rt.launch_walk_backward(per_cam2, cams, W, H, color, log_t, color, None, grads=good)
 
 
 def test_sh_prepass_buffers_are_checked_before_any_launch():
