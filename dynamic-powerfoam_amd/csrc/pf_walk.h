@@ -347,6 +347,105 @@ __device__ __forceinline__ void texel_bwd(f3 o, f3 d, float t_near_in, f3 c, f3 
     }
 }
 
+// fl(w / wsum) < 1e-3f, decided from products except within 2^-20 of the
+// boundary, where the division is done: c * wsum is within 2^-23 of the
+// exact product, so outside that band the product comparison agrees with
+// the correctly rounded quotient's (texture.py:118-119, :158-159 test the
+// quotient).
+__device__ __forceinline__ bool below_1e3(float w, float wsum) {
+    const float t = 1e-3f * wsum;
+    if (w < t * (1.0f - 0x1p-20f)) return true;
+    if (w > t * (1.0f + 0x1p-20f)) return false;
+    return w / wsum < 1e-3f;
+}
+
+// texel_bwd for a sink whose calls all 64 lanes make together
+// (Sink::kUniformHeight): both passes run on every lane (zeros where a
+// texel is skipped or no height gradient flows), and each texel site's two
+// terms (colour pass, height pass) are summed before one sink call.  The
+// arithmetic of each term is texel_bwd's.
+template <typename A, int KT, typename RGBCell, typename Sink>
+__device__ __forceinline__ void texel_bwd_uniform(f3 o, f3 d, float t_near_in, f3 c, f3 n, float r, const A *sites,
+                                                  const RGBCell &rgbs, const A *heights, const TexFwdT<KT> &fw,
+                                                  f3 dL_drgb, float dL_dtsurf_in, const Sink &sink, f3 &dL_dc,
+                                                  f3 &dL_dn, float &dL_dtnear, bool exact, float &dL_dr) {
+    static_assert(KT > 0, "fixed K");
+    dL_dc = mk(0, 0, 0);
+    dL_dn = mk(0, 0, 0);
+    float dL_dheight = 0.0f;
+    const float inv_r2 = 1.0f / (r * r);
+    const float tq = fw.dp >= 0.0f ? t_near_in : fmax_(t_near_in, fw.t_surf);
+    const f3 q = ray_at(o, tq, d);
+    f3 dL_dq = mk(0, 0, 0);
+    const float gdp_in = dot(dL_drgb, fw.rgb);
+    const float k2 = (2.0f * kTemp) * inv_r2;
+    const float inv_ws = 1.0f / fw.wsum;
+    float a[KT];  // colour-pass site coefficient: site term a_i (q - s_i)
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+        const f3 s = ld3(sites + 3 * i);
+        const f3 dv = q - s;
+        const float w = fw.w1[i];
+        const bool keep = exact || !below_1e3(w, fw.wsum);
+        const float kw = k2 * w;
+        f3 g_rgb = (w * inv_ws) * dL_drgb;
+        const float gdp = dot(dL_drgb, rgbs(i, s));
+        const float coef = (gdp - gdp_in) * inv_ws;
+        const float ai = kw * coef;
+        const f3 gs = ai * dv;
+        g_rgb = keep ? g_rgb : mk(0, 0, 0);
+        a[i] = keep ? ai : 0.0f;
+        sink.rgb(i, g_rgb);
+        if (keep) dL_dq = dL_dq + (-1.0f * gs);
+        if (exact) dL_dr += coef * (kw * dot(dv, dv) / r);
+    }
+    const float dL_dtq = dot(dL_dq, d);
+    dL_dtnear = (fw.dp >= 0.0f || t_near_in >= fw.t_surf) ? dL_dtq : 0.0f;
+    float dL_dts = (fw.dp < 0.0f && t_near_in < fw.t_surf) ? dL_dtq : 0.0f;
+    dL_dts += dL_dtsurf_in;
+    if (__builtin_fabsf(dL_dts) > 0.0f) {
+        f3 dtdp, dtdn;
+        float dtdh;
+        ray_plane_bwd(o, d, c, n, fw.height, dtdp, dtdn, dtdh);
+        dL_dc = dL_dc + dtdp * dL_dts;
+        dL_dn = dL_dn + dtdn * dL_dts;
+        dL_dheight += dtdh * dL_dts;
+    }
+    const float tq0 = fw.dp0 >= 0.0f ? t_near_in : fmax_(t_near_in, fw.t0);
+    const f3 q0 = ray_at(o, tq0, d);
+    f3 dL_dq0 = mk(0, 0, 0);
+    const float hdp_in = dL_dheight * fw.height;
+    const float inv_ws0 = 1.0f / fw.wsum0;
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+        const f3 s = ld3(sites + 3 * i);
+        const f3 dv0 = q0 - s;
+        const float w = fw.w0[i];
+        const bool keep = exact || !below_1e3(w, fw.wsum0);
+        const float kw = k2 * w;
+        float hg = (w * dL_dheight) * inv_ws0;
+        const float hdp = dL_dheight * (float)heights[i];
+        const float coef = (hdp - hdp_in) * inv_ws0;
+        f3 gs0 = (kw * coef) * dv0;
+        hg = keep ? hg : 0.0f;
+        gs0 = keep ? gs0 : mk(0, 0, 0);
+        sink.height(i, hg);
+        sink.site(i, a[i] * (q - s) + gs0);
+        dL_dq0 = dL_dq0 + (-1.0f * gs0);
+        if (exact) dL_dr += coef * (kw * dot(dv0, dv0) / r);
+    }
+    const float dL_dtq0 = dot(dL_dq0, d);
+    dL_dtnear += (fw.dp >= 0.0f || t_near_in >= fw.t0) ? dL_dtq0 : 0.0f;
+    const float dL_dts0 = (fw.dp < 0.0f && t_near_in < fw.t0) ? dL_dtq0 : 0.0f;
+    if (__builtin_fabsf(dL_dts0) > 0.0f) {
+        f3 dtdp, dtdn;
+        float dtdh;
+        ray_plane_bwd(o, d, c, n, 0.0f, dtdp, dtdn, dtdh);
+        dL_dc = dL_dc + dtdp * dL_dts0;
+        dL_dn = dL_dn + dtdn * dL_dts0;
+    }
+}
+
 // --------------------------------------------------------------- the walk --
 // Algorithmic bytes (what a visit, a face or a shaded segment must read at
 // least once): sphere 16 B + CSR row bounds 8 B + [n, sigma]; per face the
@@ -769,15 +868,26 @@ __device__ __forceinline__ uint64_t segment_grad_bytes(int K, int tn0_id, int tf
            (tf0_id >= 0 && tf0_id < n_adj ? 32ull : 0ull);
 }
 
+// The gradients a segment sends to the neighbours across its bounding
+// faces (side 0: near, 1: far): the neighbour's sphere [c, r] terms, or
+// nb = -1 when that bound is not a face.
+struct FaceTerms {
+    int nb[2];
+    f3 dc[2];
+    float dr[2];
+};
+
 // Gradients of one segment (rasterize.py:1371-1516, texture.py:74-186):
 //   dL/d(delta_s) = g.(C - C_{<=s}) + g_logT - (g.c_s) T_s exp(delta_s),
 // routed to the sphere, a power face or the texel plane.  The cell's own
 // gradients go to sink (sphere, [n, sigma], texel sites, colours,
-// heights); neighbour spheres get global atomics.
+// heights); neighbour spheres get global atomics, or, with ft, are handed
+// back for the caller to sum across lanes first.
 template <typename A, typename F, int KT, typename RGB, typename Sink>
 __device__ __forceinline__ void segment_grads_to(const SceneDev<A> &s, const RGB &colors, f3 o, f3 d, const Seg &q,
                                                  const TexFwdT<KT> &tx, f3 g, float gL, f3 Cfin, const GradDev &G,
-                                                 bool exact, const Sink &sink) {
+                                                 bool exact, const Sink &sink, FaceTerms *ft = nullptr) {
+    if (ft) ft->nb[0] = ft->nb[1] = -1;
     const int K = KT > 0 ? KT : s.K;
     const int cell = q.cell, n_adj = q.n_adj;
     const f3 c = q.c;
@@ -798,8 +908,12 @@ __device__ __forceinline__ void segment_grads_to(const SceneDev<A> &s, const RGB
     const auto rgbs = colors.cell(cell, K);
     f3 dL_dc, dL_dn;
     float dL_dtn0, dL_dr = 0.0f;
-    texel_bwd<A, KT>(o, d, q.tn0, c, q.nrm, r, s.sites + 3 * tbase, rgbs, s.height + tbase, K, tx, dL_dcolor, dL_dts,
-                     sink, dL_dc, dL_dn, dL_dtn0, exact, dL_dr);
+    if constexpr (Sink::kUniformHeight)
+        texel_bwd_uniform<A, KT>(o, d, q.tn0, c, q.nrm, r, s.sites + 3 * tbase, rgbs, s.height + tbase, tx,
+                                 dL_dcolor, dL_dts, sink, dL_dc, dL_dn, dL_dtn0, exact, dL_dr);
+    else
+        texel_bwd<A, KT>(o, d, q.tn0, c, q.nrm, r, s.sites + 3 * tbase, rgbs, s.height + tbase, K, tx, dL_dcolor,
+                         dL_dts, sink, dL_dc, dL_dn, dL_dtn0, exact, dL_dr);
     if (q.tn_id != n_adj) dL_dtn0 += dL_dtn;
     for (int side = 0; side < 2; ++side) {
         const int tid = side == 0 ? q.tn0_id : q.tf0_id;
@@ -819,11 +933,17 @@ __device__ __forceinline__ void segment_grads_to(const SceneDev<A> &s, const RGB
             const FaceGrad fg = ray_pface_bwd(o, d, c, r, mk(asp.x, asp.y, asp.z), asp.w);
             dL_dc = dL_dc + fg.dc * dLdt;
             dL_dr += dLdt * fg.dr;
-            float *ga = G.spheres + 4 * (int64_t)adjc;
-            atomic_add(ga + 0, dLdt * fg.dcj.x);
-            atomic_add(ga + 1, dLdt * fg.dcj.y);
-            atomic_add(ga + 2, dLdt * fg.dcj.z);
-            atomic_add(ga + 3, dLdt * fg.drj);
+            if (ft) {
+                ft->nb[side] = adjc;
+                ft->dc[side] = fg.dcj * dLdt;
+                ft->dr[side] = dLdt * fg.drj;
+            } else {
+                float *ga = G.spheres + 4 * (int64_t)adjc;
+                atomic_add(ga + 0, dLdt * fg.dcj.x);
+                atomic_add(ga + 1, dLdt * fg.dcj.y);
+                atomic_add(ga + 2, dLdt * fg.dcj.z);
+                atomic_add(ga + 3, dLdt * fg.drj);
+            }
         }
     }
     sink.sphere(dL_dc, dL_dr);

@@ -28,11 +28,14 @@
 
 namespace pf {
 
+#ifndef PF_GROUP_WAVES
+#define PF_GROUP_WAVES 4  // waves per SIMD of the grouped gradient kernel
+#endif
 constexpr int kGroupRows = 16;                  // LDS rows per wave (runs held at once)
 constexpr int kGroupPitch = 8 + 7 * 8 + 1;      // K = 8: 64 terms, odd pitch
 constexpr int kScanChunk = 4096;                // entries per scan block (256 x 16)
 constexpr int kReplayBlocks = 16;               // blocks replaying rays past the cap
-constexpr int kGroupBlocksPerCU = 4;
+constexpr int kGroupBlocksPerCU = PF_GROUP_WAVES;
 
 struct GroupWs {
     int *bucket_off;  // nb + 1: counts, then exclusive offsets (bucket_off[nb] = total)
@@ -68,6 +71,25 @@ __device__ __forceinline__ SlotRay slot_ray(const WalkParams &p, int64_t slot) {
     const int tile = (int)(t % p.tiles_per_cam);
     r.pix_i = (tile % p.tiles_h) * 8 + (lane & 7);
     r.pix_j = (tile / p.tiles_h) * 8 + (lane >> 3);
+    r.valid = r.pix_i < p.H && r.pix_j < p.W;
+    r.ray = ((int64_t)r.b * p.H + r.pix_i) * p.W + r.pix_j;
+    return r;
+}
+
+// The same map in 32-bit arithmetic, for slots below 2^31 (the grouped
+// backward's records hold fewer than 2^31 entries, so fewer slots).
+__device__ __forceinline__ SlotRay slot_ray32(const WalkParams &p, uint32_t slot) {
+    SlotRay r;
+    const uint32_t t = slot >> 6;
+    const uint32_t lane = slot & 63u;
+    const uint32_t tpc = (uint32_t)p.tiles_per_cam, th = (uint32_t)p.tiles_h;
+    const uint32_t b = t / tpc;
+    const uint32_t tile = t - b * tpc;
+    const uint32_t tj = tile / th;
+    const uint32_t ti = tile - tj * th;
+    r.b = (int)b;
+    r.pix_i = (int)(ti * 8 + (lane & 7));
+    r.pix_j = (int)(tj * 8 + (lane >> 3));
     r.valid = r.pix_i < p.H && r.pix_j < p.W;
     r.ray = ((int64_t)r.b * p.H + r.pix_i) * p.W + r.pix_j;
     return r;
@@ -224,6 +246,44 @@ struct RunScan {
     }
 };
 
+// Sum of x over all 64 lanes, in lane 63 (the in-row scan by row shifts,
+// then row_bcast:15 and row_bcast:31).  All 64 lanes must call it together.
+__device__ __forceinline__ float wave_total(float x) {
+    x += dpp_f<0x111, 0xF>(x);
+    x += dpp_f<0x112, 0xF>(x);
+    x += dpp_f<0x114, 0xF>(x);
+    x += dpp_f<0x118, 0xF>(x);
+    x += dpp_f<0x142, 0xA>(x);
+    x += dpp_f<0x143, 0xC>(x);
+    return x;
+}
+
+// Adds the lanes' neighbour-face terms (one side) to the neighbours'
+// sphere gradients: one round per distinct neighbour among the lanes, whose
+// terms are summed across the wave and added by one lane.  Neighbouring
+// rays through a cell leave it by few faces, so a wave has few distinct
+// neighbours per side and the atomics no longer pile up on one address.
+__device__ __forceinline__ void add_face_terms(int nb, f3 dc, float dr, float *spheres) {
+    uint64_t pend = __ballot(nb >= 0);
+    while (pend) {
+        const int lead = __ffsll((unsigned long long)pend) - 1;
+        const int tgt = __builtin_amdgcn_readlane(nb, lead);
+        const bool m = nb == tgt;
+        pend &= ~__ballot(m);
+        const float sx = wave_total(m ? dc.x : 0.0f);
+        const float sy = wave_total(m ? dc.y : 0.0f);
+        const float sz = wave_total(m ? dc.z : 0.0f);
+        const float sr = wave_total(m ? dr : 0.0f);
+        if (__lane_id() == 63) {
+            float *ga = spheres + 4 * (int64_t)tgt;
+            if (sx != 0.0f) atomic_add(ga + 0, sx);
+            if (sy != 0.0f) atomic_add(ga + 1, sy);
+            if (sz != 0.0f) atomic_add(ga + 2, sz);
+            if (sr != 0.0f) atomic_add(ga + 3, sr);
+        }
+    }
+}
+
 // Sink of the grouped pass: every term is summed over its run; the run's
 // last lane adds the total into the run's LDS row, or straight into global
 // memory when the wave holds more runs than rows.
@@ -307,8 +367,7 @@ __device__ __forceinline__ void grouped_range(const GroupParams &q, int64_t lo, 
         const int e = act ? q.ws.sorted[i] : 0;
         const int4 rq = act ? p.rec_segs[e] : make_int4(0, 0, 0, 0);
         const float4 pt = act ? q.part[e] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        const int64_t slot = ((int64_t)e >> 3) % q.S;
-        const SlotRay sr = slot_ray(p, slot);
+        const SlotRay sr = slot_ray32(p, ((uint32_t)e >> 3) % (uint32_t)q.S);
         const int cell = act ? rq.x : 0;
         const int key = act ? bucket_of(q, cell, sr.b) : -2;
         // runs of equal buckets; lane 0 continues the carried run if equal
@@ -367,10 +426,13 @@ __device__ __forceinline__ void grouped_range(const GroupParams &q, int64_t lo, 
         const GroupSink sink{scan, act, end, run < kGroupRows ? rows + run * kGroupPitch : nullptr,
                              G.spheres + 4 * (int64_t)cell, G.nsigmas + 4 * (int64_t)cell, G.sites + 3 * tbase,
                              G.rgb + (int64_t)sr.b * rgb_stride + 3 * tbase, G.height + tbase};
+        FaceTerms ft;
         segment_grads_to<A, F, 8>(s, colors, o, d,
                                   Seg{cell, c, nrm, r, sigma, tn0, dt, e_dlt, alpha, pt.x, mk(pt.y, pt.z, pt.w),
                                       tn_id, tf_id, tn0_id, tf0_id, e0, n_adj, tbase},
-                                  tx, g, gL, Cfin, G, p.exact_texel, sink);
+                                  tx, g, gL, Cfin, G, p.exact_texel, sink, &ft);
+        add_face_terms(act ? ft.nb[0] : -1, ft.dc[0], ft.dr[0], G.spheres);
+        add_face_terms(act ? ft.nb[1] : -1, ft.dc[1], ft.dr[1], G.spheres);
         if (act && p.alg_bytes) {
             constexpr uint64_t kVisit = WalkBytes<A, F>::visit, kTexel = WalkBytes<A, F>::texel;
             // record entry and partials, its sort (two cell reads, the id
@@ -455,7 +517,8 @@ __device__ __forceinline__ void replay_over(const GroupParams &q, int64_t first,
 // into equal contiguous ranges, one per wave, numbered XCD-aware so that one
 // XCD's waves take neighbouring ranges (the same cells' rows).
 template <typename A, typename F>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void grp_grad_kernel(GroupParams q) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PF_GROUP_WAVES, PF_GROUP_WAVES))) void
+grp_grad_kernel(GroupParams q) {
     __shared__ float rows[4 * kGroupRows * kGroupPitch];
     if (!record_ok(q.header)) return;
     if ((int)blockIdx.x < kReplayBlocks) {
