@@ -117,29 +117,59 @@ __global__ void grp_zero_kernel(GroupParams q) {
 
 // One thread per slot: rays past the cap go to the replay list; the others
 // count their recorded segments per bucket.  FILL: scatter each segment's
-// record entry id to its bucket's next position instead.
+// record entry id to its bucket's next position instead.  Neighbouring
+// rays (lanes of a tile) record the same cells at the same step, so the
+// lanes of one bucket first agree on a leader (one ballot round per
+// distinct bucket), and each leader makes one atomic for all of them;
+// with FILL the leader's returned base plus the lane's rank among them is
+// its position.
 template <bool FILL> __global__ __launch_bounds__(256) void grp_bucket_kernel(GroupParams q) {
     const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= q.S || !record_ok(q.header)) return;
+    const bool ok = record_ok(q.header);
     const WalkParams &p = q.w;
-    const SlotRay sr = slot_ray(p, slot);
-    if (!sr.valid || !ray_live(p, sr.ray)) return;
-    const int count = p.rec_counts[slot];
-    if (count > p.rec_cap) {
-        if (!FILL) q.ws.over_list[atomicAdd(q.ws.over_count, 1)] = (int)slot;
-        return;
+    const SlotRay sr = slot_ray(p, slot < q.S ? slot : 0);
+    int count = 0;
+    if (ok && slot < q.S && sr.valid && ray_live(p, sr.ray)) {
+        count = p.rec_counts[slot];
+        if (count > p.rec_cap) {
+            if (!FILL) q.ws.over_list[atomicAdd(q.ws.over_count, 1)] = (int)slot;
+            count = 0;
+        }
     }
     const int N = (int)p.s.num_cells;
-    for (int k = 0; k < count; ++k) {
-        const int64_t e = rec_index(k, slot, q.S);
-        const int cell = p.rec_segs[e].x;
-        if ((unsigned)cell >= (unsigned)N) {
-            if (!FILL && p.status) atomicOr(p.status, PF_STATUS_BAD_RECORD);
-            continue;
+    const int lane = __lane_id();
+    const uint64_t lt = (1ull << lane) - 1;
+    for (int k = 0; __any(k < count); ++k) {
+        int bk = -1;
+        int64_t e = 0;
+        if (k < count) {
+            e = rec_index(k, slot, q.S);
+            const int cell = p.rec_segs[e].x;
+            if ((unsigned)cell < (unsigned)N) bk = bucket_of(q, cell, sr.b);
+            else if (!FILL && p.status) atomicOr(p.status, PF_STATUS_BAD_RECORD);
         }
-        const int bk = bucket_of(q, cell, sr.b);
-        if (FILL) q.ws.sorted[atomicAdd(q.ws.cursor + bk, 1)] = (int)e;
-        else atomicAdd(q.ws.bucket_off + bk, 1);
+        uint64_t pend = __ballot(bk >= 0);
+        int leader = 0, rank = 0, n = 0;
+        while (pend) {
+            const int lead = __ffsll((unsigned long long)pend) - 1;
+            const int b = __builtin_amdgcn_readlane(bk, lead);
+            const uint64_t mem = __ballot(bk == b);
+            pend &= ~mem;
+            if (bk == b) {
+                leader = lead;
+                rank = __builtin_popcountll(mem & lt);
+                n = __builtin_popcountll(mem);
+            }
+        }
+        const bool lead_here = bk >= 0 && leader == lane;
+        if (FILL) {
+            int base = 0;
+            if (lead_here) base = atomicAdd(q.ws.cursor + bk, n);
+            base = __shfl(base, leader, 64);
+            if (bk >= 0) q.ws.sorted[base + rank] = (int)e;
+        } else if (lead_here) {
+            atomicAdd(q.ws.bucket_off + bk, n);
+        }
     }
 }
 
