@@ -181,8 +181,8 @@ class RenderServer:
         # forward+backward tiles record their contributing segments (up to
         # record_cap per ray, a multiple of 8; 0 replays the walk) in one
         # workspace per table buffer, allocated with the first such request:
-        # 16 * record_cap * 64 bytes per tile of capacity and buffer, fixed
-        # server memory outside memory_budget (6.4 GB at the defaults)
+        # 16 * record_cap * 64 bytes per tile of capacity and buffer (6.4 GB
+        # at the defaults), reserved from memory_budget
         self.record_cap = -(-int(record_cap) // 8) * 8 if record_cap > 0 else 0
         self._records = None
         self.record_bytes = 0
@@ -201,7 +201,13 @@ class RenderServer:
         n = len(self.scenes)
         arr = (_lib.PfServeScene * n)(*[s.c for s in self.scenes.values()])
         raw = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8)
-        self._scene_tab = raw.to(self.device)
+        new_tab = raw.to(self.device)
+        # steps in flight on the server's streams read the old table: it is
+        # released only once they have finished
+        for ev in self._events:
+            if ev is not None:
+                ev.synchronize()
+        self._scene_tab = new_tab
         self._graphs = None  # the scene table pointer changed
 
     # ---------------------------------------------------------- requests --
@@ -218,9 +224,17 @@ class RenderServer:
         return b
 
     def _ensure_records(self):
+        """The forward+backward tiles' record workspaces, allocated with the
+        first such request and counted against the memory budget; if they do
+        not fit, those tiles replay their walks instead (record_cap 0)."""
         if self._records is not None or self.record_cap <= 0:
             return
         slots = _lib.load().pf_serve_record_slots(self.capacity)
+        need = 16 * self.record_cap * slots * self.nbuf
+        if self.reserved + need > self.budget:
+            self.record_cap = 0
+            return
+        self.reserved += need
         self._records = []
         for _ in range(self.nbuf):
             segs = torch.empty(self.record_cap * slots, 4, dtype=torch.int32, device=self.device)
@@ -242,7 +256,7 @@ class RenderServer:
             self._ensure_records()
         r._bytes = self._request_bytes(r)
         r._submitted = time.perf_counter()
-        if r._bytes > self.budget - sum(s.nbytes() for s in self.scenes.values()):
+        if r._bytes > self.budget - sum(s.nbytes() for s in self.scenes.values()) - self.record_bytes:
             r.rejected = True
             self.rejected.append(r)
             return False

@@ -5,10 +5,12 @@
   oracle and must match bit for bit (visits, sequence hashes, colour, log T),
   and the backward of an L2 loss on those rays must match the oracle's
   within 1e-3 relative L2 per gradient array.
-* 262,144 cells, 16 cameras at 800x800 sharing one set of texel colours:
-  the backward (exact texel mode, f32 faces) of a loss on 1,024 sampled
-  rays against the oracle's backward on every parameter, and against FP64
-  central differences (step 1e-4) of the FP64 forward on 64 random scalars.
+* 262,144 cells, 16 cameras at 800x800, each with its own texel colours
+  (its SH pre-pass): with fp16 faces, the grouped, recorded and replay
+  backwards of a loss on 1,024 sampled rays against the oracle's backward
+  on every parameter; with f32 faces in exact texel mode, also against
+  FP64 central differences (step 1e-4) of the FP64 forward on 64 random
+  scalars.
   A scalar within a step of a jump (an exit face switches) or a bend (the
   min / max of two faces' t switches) is drawn again: there the central
   quotient disagrees with the half-step one or the one-sided slopes with
@@ -106,14 +108,34 @@ def test_1m_cells_1080p_sampled_rays():
         assert _rel(grec[k].cpu().numpy(), gg[k].cpu().numpy()) < 1e-5, k
 
 
-def test_262k_cells_16_cameras_finite_differences():
+def _per_camera_oracle_backward(sc, adj, off, faces, rgb, words, color, gc, gl, exact):
+    """The oracle's backward of a batch whose cameras have their own texel
+    colours: one camera at a time, shared arrays summed, colour gradients
+    per camera."""
+    B = words.shape[0]
+    out = {k: 0.0 for k in ("spheres", "nsigmas", "texel_sites", "texel_height")}
+    out["texel_rgb"] = []
+    for b in range(B):
+        cs = _cpu_scene(sc, adj, off, faces, rgb[b])
+        cb = orc.walk_backward(cs, words[b:b + 1], color[b:b + 1], gc[b:b + 1], gl[b:b + 1], exact=exact)
+        for k in ("spheres", "nsigmas", "texel_sites", "texel_height"):
+            out[k] = out[k] + cb[k]
+        out["texel_rgb"].append(cb["texel_rgb"])
+    out["texel_rgb"] = np.stack(out["texel_rgb"])
+    return out
+
+
+def _c3_batch(f16):
+    """262,144 cells, 16 cameras at 800x800, each camera with the texel
+    colours of its own SH pre-pass (texel_rgb (16, N, K, 3)), and an L2 +
+    alpha loss on 1,024 sampled rays."""
     W = H = 800
     n, B = 262_144, 16
     sc, adj, off, faces, words, cams_dev, rgb, _ = _setup(n, lambda r: workloads.sphere_cameras(r, B, W, H), 1,
-                                                         f16=False)
-    # one set of texel colours for all cameras (the SH pre-pass of camera 0)
-    gs = _gpu_scene(sc, adj, off, faces, rgb[0], 1, n)
-    fwd = rt.launch_walk_forward(gs, cams_dev, W, H)
+                                                         f16=f16)
+    gs = _gpu_scene(sc, adj, off, faces, rgb, B, n)
+    assert gs.rgb_cams == B
+    fwd = rt.launch_walk_forward(gs, cams_dev, W, H, record_cap=32)
     rng = np.random.default_rng(5)
     rays = np.sort(rng.choice(B * W * H, 1024, replace=False))
     target = rng.uniform(0, 1, (B, H, W, 3)).astype(np.float32)
@@ -122,53 +144,101 @@ def test_262k_cells_16_cameras_finite_differences():
     mask[torch.from_numpy(rays).to(DEV)] = 1.0
     gc = 2.0 * (fwd["color"] - torch.from_numpy(target).to(DEV)) * mask.view(B, H, W, 1)
     gl = -w_alpha * torch.exp(fwd["log_t"]) * mask.view(B, H, W)
-    g = rt.launch_walk_backward(gs, cams_dev, W, H, fwd["color"], fwd["log_t"], gc, gl, exact_texel=True)
-    # the recorded backward at a cap some rays exceed (both halves of its launch)
-    fr = rt.launch_walk_forward(gs, cams_dev, W, H, record_cap=32)
-    gr = rt.launch_walk_backward(gs, cams_dev, W, H, fwd["color"], fwd["log_t"], gc, gl, exact_texel=True,
-                                 record=fr["record"])
+    return sc, adj, off, faces, words, cams_dev, rgb, gs, fwd, rays, target, w_alpha, gc, gl, (W, H, n, B)
+
+
+def test_262k_cells_16_cameras_per_camera_colours_fp16_faces():
+    """The differentiation configuration as benchmarked: fp16 (packed) walk
+    faces, per-camera colours, the reference's texel approximations; the
+    grouped backward (per-camera colour buckets), the ray-by-ray recorded
+    backward and the replay against the oracle on every parameter."""
+    sc, adj, off, faces, words, cams_dev, rgb, gs, fwd, rays, target, w_alpha, gc, gl, dims = _c3_batch(True)
+    W, H, n, B = dims
+    grp = rt.launch_walk_backward(gs, cams_dev, W, H, fwd["color"], fwd["log_t"], gc, gl, record=fwd["record"],
+                                  grouped=True)
+    rec = rt.launch_walk_backward(gs, cams_dev, W, H, fwd["color"], fwd["log_t"], gc, gl, record=fwd["record"],
+                                  grouped=False)
+    rep = rt.launch_walk_backward(gs, cams_dev, W, H, fwd["color"], fwd["log_t"], gc, gl)
     torch.cuda.synchronize()
-    assert bool((fr["record"]["counts"] > 32).any())
+    assert tuple(grp["texel_rgb"].shape) == (B, n, 8, 3)
+    assert int(grp["status"].item()) & ~rt_status_cycle() == 0
+    cb = _per_camera_oracle_backward(sc, adj, off, faces, rgb.cpu().numpy(), words, fwd["color"].cpu().numpy(),
+                                     gc.cpu().numpy(), gl.cpu().numpy(), exact=False)
+    for k in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
+        assert _rel(grp[k].cpu().numpy(), cb[k]) < 1e-3, k
+        assert _rel(grp[k].cpu().numpy(), rep[k].cpu().numpy()) < 1e-5, k
+        assert _rel(rec[k].cpu().numpy(), rep[k].cpu().numpy()) < 1e-5, k
+
+
+def test_262k_cells_16_cameras_finite_differences():
+    """Exact texel mode, f32 faces (so that the FP64 forward's exact faces
+    are the kernels' faces to rounding), per-camera colours: the grouped and
+    recorded backwards against the replay and the oracle on every
+    parameter, and against FP64 central differences on 64 scalars."""
+    sc, adj, off, faces, words, cams_dev, rgb, gs, fwd, rays, target, w_alpha, gc, gl, dims = _c3_batch(False)
+    W, H, n, B = dims
+    g = rt.launch_walk_backward(gs, cams_dev, W, H, fwd["color"], fwd["log_t"], gc, gl, exact_texel=True)
+    gr = rt.launch_walk_backward(gs, cams_dev, W, H, fwd["color"], fwd["log_t"], gc, gl, exact_texel=True,
+                                 record=fwd["record"], grouped=True)
+    grr = rt.launch_walk_backward(gs, cams_dev, W, H, fwd["color"], fwd["log_t"], gc, gl, exact_texel=True,
+                                  record=fwd["record"], grouped=False)
+    torch.cuda.synchronize()
+    assert bool((fwd["record"]["counts"] > 32).any())  # some rays replayed in the same launch
     for k in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
         assert _rel(gr[k].cpu().numpy(), g[k].cpu().numpy()) < 1e-5, k
-    del fr, gr
+        assert _rel(grr[k].cpu().numpy(), g[k].cpu().numpy()) < 1e-5, k
     g = {k: v.cpu().numpy() for k, v in g.items() if v is not None and k != "status"}
-    # the oracle's backward on every parameter
-    rgb0 = rgb[0].cpu().numpy()
-    cs = _cpu_scene(sc, adj, off, faces, rgb0)
-    cb = orc.walk_backward(cs, words, fwd["color"].cpu().numpy(), gc.cpu().numpy(), gl.cpu().numpy(), exact=True)
+    rgb_np = rgb.cpu().numpy()
+    cb = _per_camera_oracle_backward(sc, adj, off, faces, rgb_np, words, fwd["color"].cpu().numpy(),
+                                     gc.cpu().numpy(), gl.cpu().numpy(), exact=True)
     for k in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
         assert _rel(g[k], cb[k]) < 1e-3, k
-    # FP64 central differences of the loss on the sampled rays
+    # FP64 central differences of the loss on the sampled rays, camera by
+    # camera with that camera's colours
     P = dict(points=sc["points"].astype(np.float64), radii=sc["radii"].astype(np.float64),
              nsigmas=np.concatenate([sc["normals"], sc["density"][:, None]], 1).astype(np.float64),
-             texel_sites=sc["texel_sites"].astype(np.float64), texel_rgb=rgb0.astype(np.float64),
-             texel_height=sc["texel_height"].astype(np.float64))
+             texel_sites=sc["texel_sites"].astype(np.float64), texel_height=sc["texel_height"].astype(np.float64))
+    rgb64 = rgb_np.astype(np.float64)
     an = dict(points=g["spheres"][:, :3], radii=g["spheres"][:, 3], nsigmas=g["nsigmas"],
               texel_sites=g["texel_sites"], texel_rgb=g["texel_rgb"], texel_height=g["texel_height"])
     adj_np, off_np = adj.cpu().numpy(), off.cpu().numpy()
+    per = W * H
+    cam_rays = [rays[(rays >= b * per) & (rays < (b + 1) * per)] - b * per for b in range(B)]
 
     def loss():
-        return orc.walk_loss_f64(P, adj_np, off_np, words, target, w_alpha=w_alpha, rays=rays)
+        tot = 0.0
+        for b in range(B):
+            if cam_rays[b].size == 0:
+                continue
+            Pb = dict(P, texel_rgb=rgb64[b])
+            tot += orc.walk_loss_f64(Pb, adj_np, off_np, words[b:b + 1], target[b:b + 1], w_alpha=w_alpha,
+                                     rays=cam_rays[b])
+        return tot
 
     touched = np.flatnonzero(np.abs(g["nsigmas"][:, 3]) > 1e-6)
-    names = list(P)
+    names = list(P) + ["texel_rgb"]
     a, fd = [], []
     tries = 0
     while len(a) < 64 and tries < 256:
         name = names[tries % len(names)]
         tries += 1
-        cell = int(rng.choice(touched))
-        idx = (cell,) + tuple(int(rng.integers(0, s)) for s in P[name].shape[1:])
-        x0 = P[name][idx]
+        cell = int(rng_choice(touched, tries))
+        if name == "texel_rgb":
+            arr = rgb64
+            b = int(np.random.default_rng(tries).integers(0, B))
+            idx = (b, cell) + tuple(int(np.random.default_rng(tries + 1).integers(0, s)) for s in arr.shape[2:])
+        else:
+            arr = P[name]
+            idx = (cell,) + tuple(int(np.random.default_rng(tries + 1).integers(0, s)) for s in arr.shape[1:])
+        x0 = arr[idx]
         l0 = loss()
         v = {}
         for h in (1e-4, 5e-5):
-            P[name][idx] = x0 + h
+            arr[idx] = x0 + h
             v[h] = loss()
-            P[name][idx] = x0 - h
+            arr[idx] = x0 - h
             v[-h] = loss()
-            P[name][idx] = x0
+            arr[idx] = x0
         central = (v[1e-4] - v[-1e-4]) / 2e-4
         half = (v[5e-5] - v[-5e-5]) / 1e-4
         right, left = (v[1e-4] - l0) / 1e-4, (l0 - v[-1e-4]) / 1e-4
@@ -183,3 +253,7 @@ def test_262k_cells_16_cameras_finite_differences():
         fd.append(central)
     assert len(a) == 64, f"only {len(a)} of {tries} scalars were away from a jump or bend"
     assert _rel(np.array(a), np.array(fd)) < 1e-3
+
+
+def rng_choice(arr, k):
+    return arr[np.random.default_rng(1000 + k).integers(0, arr.size)]
