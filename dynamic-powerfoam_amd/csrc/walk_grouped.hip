@@ -26,6 +26,8 @@
 #include "pf_walk.h"
 #include "pf_walk_params.h"
 
+#include <mutex>
+
 namespace pf {
 
 #ifndef PF_GROUP_WAVES
@@ -614,6 +616,16 @@ extern "C" int64_t pf_walk_grouped_workspace_bytes(int64_t num_cells, int32_t nu
 int pf_walk_check_stamp(const pf_walk_record *record, const pf_scene *scene, const pf_camera *cams,
                         int32_t num_cams, int64_t total_tiles, int32_t *status, hipStream_t st);
 
+// The grouped backward with its workspace; workspace == nullptr: the
+// library's static scratch array of this device (the no-allocation entry
+// point), whose address is looked up here.
+static int grouped_backward(const pf_scene *scene, const pf_camera *cams, int32_t num_cams, int32_t width,
+                            int32_t height, float transmittance_threshold, int32_t max_steps, const float *color,
+                            const float *log_t, const float *grad_color, const float *grad_log_t,
+                            const pf_walk_record *record, const pf_grads *grads, int32_t flags, void *workspace,
+                            int64_t workspace_bytes, bool use_scratch, int32_t *status, uint64_t *alg_bytes,
+                            void *stream);
+
 extern "C" int pf_walk_backward_grouped(const pf_scene *scene, const pf_camera *cams, int32_t num_cams,
                                         int32_t width, int32_t height, float transmittance_threshold,
                                         int32_t max_steps, const float *color, const float *log_t,
@@ -621,6 +633,58 @@ extern "C" int pf_walk_backward_grouped(const pf_scene *scene, const pf_camera *
                                         const pf_walk_record *record, const pf_grads *grads, int32_t flags,
                                         void *workspace, int64_t workspace_bytes, int32_t *status,
                                         uint64_t *alg_bytes, void *stream) {
+    return grouped_backward(scene, cams, num_cams, width, height, transmittance_threshold, max_steps, color, log_t,
+                            grad_color, grad_log_t, record, grads, flags, workspace, workspace_bytes, false, status,
+                            alg_bytes, stream);
+}
+
+// ------------------------------------------------ no-allocation variant --
+// A file-scope device array serves as the grouped backward's workspace for
+// callers that cannot allocate (code under HIP graph capture, serving
+// loops, harnesses that forbid allocation).  Each GPU holds its own copy
+// until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns with that device current (the array's name is
+// a host-side stand-in, not a device address), looked up once per device.
+// Every call on one GPU shares that GPU's array: the kernels clear what
+// must start at zero, and calls that overlap on one GPU race.
+constexpr int64_t kScratchBytes = int64_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static void *g_scratch[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_for_current_device(void **ptr) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return PF_ERR_HIP;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return PF_ERR_HIP;
+        g_scratch[dev] = p;
+    }
+    *ptr = g_scratch[dev];
+    return PF_OK;
+}
+
+extern "C" int64_t pf_walk_grouped_scratch_bytes(void) { return kScratchBytes; }
+
+extern "C" int pf_walk_backward_grouped_noalloc(const pf_scene *scene, const pf_camera *cams, int32_t num_cams,
+                                                int32_t width, int32_t height, float transmittance_threshold,
+                                                int32_t max_steps, const float *color, const float *log_t,
+                                                const float *grad_color, const float *grad_log_t,
+                                                const pf_walk_record *record, const pf_grads *grads, int32_t flags,
+                                                int32_t *status, uint64_t *alg_bytes, void *stream) {
+    return grouped_backward(scene, cams, num_cams, width, height, transmittance_threshold, max_steps, color, log_t,
+                            grad_color, grad_log_t, record, grads, flags, nullptr, 0, true, status, alg_bytes,
+                            stream);
+}
+
+static int grouped_backward(const pf_scene *scene, const pf_camera *cams, int32_t num_cams, int32_t width,
+                            int32_t height, float transmittance_threshold, int32_t max_steps, const float *color,
+                            const float *log_t, const float *grad_color, const float *grad_log_t,
+                            const pf_walk_record *record, const pf_grads *grads, int32_t flags, void *workspace,
+                            int64_t workspace_bytes, bool use_scratch, int32_t *status, uint64_t *alg_bytes,
+                            void *stream) {
     if (!scene || !cams || num_cams <= 0 || width <= 0 || height <= 0 || !color || !grads || max_steps <= 0)
         return PF_ERR_ARG;
     if (!scene->spheres || !scene->nsigmas || !scene->texel_sites || !scene->texel_rgb || !scene->texel_height ||
@@ -637,6 +701,13 @@ extern "C" int pf_walk_backward_grouped(const pf_scene *scene, const pf_camera *
     const bool per_camera = grads->texel_rgb_cam_stride != 0 && num_cams > 1;
     GroupLayout L;
     if (!group_layout(scene->num_cells, num_cams, width, height, record->cap, per_camera, L)) return PF_ERR_ARG;
+    if (use_scratch) {
+        // sizes past the static array are refused before any device work
+        if (L.total > kScratchBytes) return PF_ERR_OVERFLOW;
+        const int rc = scratch_for_current_device(&workspace);
+        if (rc) return rc;
+        workspace_bytes = kScratchBytes;
+    }
     if (!workspace || workspace_bytes < L.total) return PF_ERR_OVERFLOW;
     if (scene->attr_dtype != PF_F32 && scene->attr_dtype != PF_F16) return PF_ERR_DTYPE;
     if (scene->face_dtype != PF_F32 && scene->face_dtype != PF_F16 && scene->face_dtype != PF_F16_PACKED)

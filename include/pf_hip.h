@@ -252,6 +252,33 @@ int pf_walk_backward_grouped(const pf_scene *scene, const pf_camera *cams,
                              int32_t *status, uint64_t *alg_bytes,
                              void *stream);
 
+/* pf_walk_backward_grouped without a workspace argument: the library's
+ * static device scratch array (pf_walk_grouped_scratch_bytes bytes, 64 MiB,
+ * one per GPU, held until the process ends) serves as the workspace, and
+ * the call allocates nothing.  No counterpart in the reference; it is for
+ * callers that cannot allocate (code under HIP graph capture, serving
+ * loops, harnesses that forbid allocation).  Sizes whose workspace
+ * (pf_walk_grouped_workspace_bytes) exceeds the scratch array return
+ * PF_ERR_OVERFLOW before any device work.  All calls on one GPU share its
+ * array: calls that overlap on one GPU race.  Other arguments and results
+ * as pf_walk_backward_grouped.                                             */
+int pf_walk_backward_grouped_noalloc(const pf_scene *scene,
+                                     const pf_camera *cams, int32_t num_cams,
+                                     int32_t width, int32_t height,
+                                     float transmittance_threshold,
+                                     int32_t max_steps, const float *color,
+                                     const float *log_t,
+                                     const float *grad_color,
+                                     const float *grad_log_t,
+                                     const pf_walk_record *record,
+                                     const pf_grads *grads, int32_t flags,
+                                     int32_t *status, uint64_t *alg_bytes,
+                                     void *stream);
+
+/* Size in bytes of the static scratch array of
+ * pf_walk_backward_grouped_noalloc.                                        */
+int64_t pf_walk_grouped_scratch_bytes(void);
+
 /* Walk face precompute (benchmark.py:295-302): for every CSR edge i->j,
  * faces[e] = half(p_j - p_i, pm_j - pm_i) with pm = 0.5(|p|^2 - r^2).
  * face_dtype PF_F16 (reference), PF_F32 or PF_F16_PACKED.                  */
