@@ -184,11 +184,13 @@ __device__ __forceinline__ FaceGrad ray_pface_bwd(f3 o, f3 d, f3 c, float r, f3 
     const float num = off - dot(o, n);
     const float dp = dot(d, n);
     FaceGrad g;
-    const f3 nd = divs(divs(num * d, dp), dp);
-    g.dc = divs(o - c, dp) + nd;
-    g.dcj = divs(cj - o, dp) - nd;
-    g.dr = r / dp;
-    g.drj = -rj / dp;
+    // gradients carry a tolerance, not bit-exactness: one division
+    const float inv = 1.0f / dp;
+    const f3 nd = ((num * inv) * inv) * d;
+    g.dc = (o - c) * inv + nd;
+    g.dcj = (cj - o) * inv - nd;
+    g.dr = r * inv;
+    g.drj = -rj * inv;
     return g;
 }
 
