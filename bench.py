@@ -8,9 +8,16 @@ lifted-hull power-diagram build, fp16 walk faces, one 1920x1080 pinhole
 camera (60 deg) on the radius-3 sphere, per-pixel L2 loss against a uniform
 random target.
 
-One step = SH colour pre-pass + walk forward + loss gradient + walk backward
-(checkpoint and replay) + SH backward, inputs resident in HBM.  Prints ONE
-JSON line.  Single GPU: --gpus above 1 exits with a message.
+One step = SH colour pre-pass + walk forward (recording each ray's
+contributing segments, their transmittance and running colour) + loss
+gradient + walk backward (the recorded segments sorted by cell and
+differentiated in cell order; rays past the record's cap replayed from the
+checkpoint in the same launch) + SH backward, inputs resident in HBM.  The
+same step with the record read ray by ray, and with checkpoint-and-replay
+and no record, is timed beside it ("backward_forms").  Also reported: the
+16-camera differentiation configuration (262,144 cells, 800x800), the tile
+rasterizer on the same frame, and the two serving traces.  Prints ONE JSON
+line.  Single GPU: --gpus above 1 exits with a message.
 """
 import argparse
 import json
@@ -61,6 +68,8 @@ def main():
     ap.add_argument("--backward", choices=("grouped", "recorded", "replay"), default="grouped",
                     help="walk backward: the record's segments sorted by cell (default), the record ray by ray, "
                          "or checkpoint-and-replay without a record")
+    ap.add_argument("--c3", type=int, default=1,
+                    help="also time the 16-camera differentiation configuration (262,144 cells, 800x800, 0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -134,11 +143,13 @@ def main():
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     times = {"fwd": [], "bwd": []}
 
-    def step(record):
+    def step(record, kind=None):
+        kind = kind or args.backward
+        cap = 0 if kind == "replay" else args.record_cap
         shading.sh_forward(coeffs, 3, sites, cams_dev, out=texel_rgb)
         if record:
             ev[0].record()
-        f = rt.launch_walk_forward(scene, cams_dev, W, H, want_depth=False, record_cap=args.record_cap)
+        f = rt.launch_walk_forward(scene, cams_dev, W, H, want_depth=False, record_cap=cap)
         if record:
             ev[1].record()
         gc = 2.0 * (f["color"] - target)
@@ -148,7 +159,8 @@ def main():
         if record:
             ev[2].record()
         b = rt.launch_walk_backward(scene, cams_dev, W, H, f["color"], f["log_t"], gc, None, grads=grads,
-                                    record=f["record"], grouped=grouped, workspace=workspace)
+                                    record=f["record"], grouped=kind == "grouped",
+                                    workspace=workspace if kind == "grouped" else None)
         if record:
             ev[3].record()
         shading.sh_backward(coeffs, 3, sites, cams_dev, grads["texel_rgb"][None], grad_coeffs=g_coeffs)
@@ -173,6 +185,28 @@ def main():
     dt = (time.perf_counter() - s0) / args.steps
     rays = W * H
     mrays = rays / dt / 1e6
+
+    # the other backward forms on the same step: the record read ray by ray,
+    # and checkpoint-and-replay without a record (no segment stored)
+    forms = {}
+    for kind in ("grouped", "recorded", "replay"):
+        if kind != "grouped" and args.record_cap <= 0:
+            continue
+        for _ in range(2):
+            step(False, kind)
+        fw, bw, st = [], [], []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            a0 = time.perf_counter()
+            step(True, kind)
+            torch.cuda.synchronize()
+            st.append((time.perf_counter() - a0) * 1e3)
+            fw.append(ev[0].elapsed_time(ev[1]))
+            bw.append(ev[2].elapsed_time(ev[3]))
+        forms[kind] = {"fwd_ms": round(float(np.median(fw)), 3), "bwd_ms": round(float(np.median(bw)), 3),
+                       "step_ms": round(float(np.median(st)), 3),
+                       "mrays_per_s": round(W * H / (float(np.median(st)) * 1e-3) / 1e6, 2)}
+    log(f"backward forms: {forms}")
 
     # algorithmic bytes from the kernels' own per-ray counters (untimed pass)
     shading.sh_forward(coeffs, 3, sites, cams_dev, out=texel_rgb)
@@ -212,6 +246,8 @@ def main():
     raster = None
     if args.raster:
         raster = raster_bench(sc, pts, rad, texel_rgb[0], target, cam, args)
+
+    c3 = c3_bench(args) if args.c3 else None
 
     serving = serving_multi = None
     if args.serve_requests > 0:
@@ -254,6 +290,7 @@ def main():
             "walk_bwd_alg_bytes": bwd_bytes,
             "record_segments": segs,
             "status": status,
+            "backward_forms": forms,
         },
         "roofline": {
             "bound": "hbm",
@@ -266,10 +303,102 @@ def main():
         },
         "cpu_baseline": cpu,
         "raster": raster,
+        "differentiation_16_cameras": c3,
         "serving": serving,
         "serving_multi": serving_multi,
     }
     print(json.dumps(out), flush=True)
+
+
+def c3_bench(args):
+    """The differentiation configuration: 262,144 GMM cells, 16 cameras of
+    800x800 on the radius-3 sphere, each with its own SH pre-pass colours,
+    walk forward recording segments, the grouped backward (per-camera colour
+    gradients) and the SH backward, an L2 loss against random targets."""
+    import numpy as np
+    import torch
+
+    from powerfoam import raytrace as rt
+    from powerfoam import shading, workloads
+    from powerfoam.camera import cameras_to_device
+    from powerfoam.geometry import morton_sort
+    from powerfoam.power import build_power_adjacency_gpu
+
+    n, B, W, H = 262_144, 16, 800, 800
+    sc = workloads.gmm_scene(n, seed=args.seed + 7)
+    cams = workloads.sphere_cameras(sc["rng"], B, W, H)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    perm = morton_sort(t(sc["points"])).long().cpu().numpy()
+    for k in ("points", "radii", "density", "normals", "texel_sites", "texel_height", "sh"):
+        sc[k] = np.ascontiguousarray(sc[k][perm])
+    pts, rad = t(sc["points"]), t(sc["radii"])
+    adj, off = build_power_adjacency_gpu(pts, rad)
+    faces = rt.walk_faces(pts, rad, adj, off, torch.float16)
+    cams_dev = cameras_to_device(cams, None, "cuda")
+    rt.start_cells_device(pts, rad, cams_dev)
+    sites, coeffs = t(sc["texel_sites"]), t(sc["sh"])
+    K = sites.shape[1]
+    texel_rgb = torch.empty(B, n, K, 3, device="cuda")
+    shading.sh_forward(coeffs, 3, sites, cams_dev, out=texel_rgb)
+    scene = rt.WalkScene(pts, rad, t(sc["density"]), t(sc["normals"]), sites, texel_rgb, t(sc["texel_height"]), adj,
+                         off, faces)
+    cap = rt.RayTracer(None, "cuda").record_cap_for(B, W, H)
+    workspace = rt.grouped_workspace(n, B, W, H, cap, "cuda")
+    target = torch.rand(B, H, W, 3, generator=torch.Generator().manual_seed(args.seed + 8)).to("cuda")
+    grads = dict(spheres=torch.zeros(n, 4, device="cuda"), nsigmas=torch.zeros(n, 4, device="cuda"),
+                 texel_sites=torch.zeros(n, K, 3, device="cuda"), texel_rgb=torch.zeros(B, n, K, 3, device="cuda"),
+                 texel_height=torch.zeros(n, K, device="cuda"))
+    g_coeffs = torch.zeros_like(coeffs)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+
+    def step(want_bytes=False):
+        shading.sh_forward(coeffs, 3, sites, cams_dev, out=texel_rgb)
+        ev[0].record()
+        f = rt.launch_walk_forward(scene, cams_dev, W, H, want_depth=False, record_cap=cap, want_bytes=want_bytes)
+        ev[1].record()
+        gc = 2.0 * (f["color"] - target)
+        for k in GRAD_KEYS:
+            grads[k].zero_()
+        g_coeffs.zero_()
+        ev[2].record()
+        b = rt.launch_walk_backward(scene, cams_dev, W, H, f["color"], f["log_t"], gc, None, grads=grads,
+                                    record=f["record"], grouped=True, workspace=workspace, want_bytes=want_bytes)
+        ev[3].record()
+        shading.sh_backward(coeffs, 3, sites, cams_dev, grads["texel_rgb"], grad_coeffs=g_coeffs)
+        return f, b
+
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    fw, bw = [], []
+    for _ in range(3):
+        step()
+        torch.cuda.synchronize()
+        fw.append(ev[0].elapsed_time(ev[1]))
+        bw.append(ev[2].elapsed_time(ev[3]))
+    reps = 5
+    torch.cuda.synchronize()
+    s0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - s0) / reps
+    f, b = step(want_bytes=True)
+    torch.cuda.synchronize()
+    fwd_bytes, bwd_bytes = float(f["alg_bytes"].sum().item()), float(b["alg_bytes"].sum().item())
+    fwd_ms, bwd_ms = float(np.median(fw)), float(np.median(bw))
+    dom_bytes, dom_ms, dom = (fwd_bytes, fwd_ms, "walk_fwd") if fwd_ms >= bwd_ms else (bwd_bytes, bwd_ms, "walk_bwd")
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    rays = B * W * H
+    out = {"workload": f"{n} GMM cells, {B} cameras {W}x{H}, per-camera SH3 pre-pass, walk fwd (record cap {cap}) "
+                       f"+ grouped bwd + SH bwd, L2 loss",
+           "ms_per_step": round(dt * 1e3, 3), "mrays_per_s": round(rays / dt / 1e6, 2), "walk_fwd_ms": round(fwd_ms, 3),
+           "walk_bwd_ms": round(bwd_ms, 3), "walk_fwd_alg_bytes": fwd_bytes, "walk_bwd_alg_bytes": bwd_bytes,
+           "record_cap": cap, "status": int(f["status"].item()) | int(b["status"].item()),
+           "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4)}}
+    log(f"16 cameras: step {dt * 1e3:.2f} ms = {out['mrays_per_s']} Mrays/s; fwd {fwd_ms:.2f} ms, bwd {bwd_ms:.2f} ms")
+    return out
 
 
 def raster_bench(sc, pts, rad, texel_rgb, target, cam, args):
@@ -324,9 +453,35 @@ def raster_bench(sc, pts, rad, texel_rgb, target, cam, args):
             ms.append(ev[0].elapsed_time(ev[1]))
         times[bwd] = float(np.median(ms))
     W, H = cam.width, cam.height
+    # algorithmic bytes, from the forward's own tile lists and stop positions:
+    # per (tile, listed cell up to the tile's stop) the cell's sphere and
+    # [n, sigma] 32 B, CSR bounds 8 B, fp16 faces 8 B each and K = 8 texels
+    # 224 B, staged once per 8x8 tile, plus the list entry 4 B; per pixel
+    # the outputs (colour, log T, normal distance, normal: 32 B); the
+    # binning reads each sphere (16 B) and writes and sorts 12 B per entry.
+    # The backward reads the same per (tile, cell) plus the neighbour ids
+    # (4 B per face), and read-modify-writes the cell's gradient row
+    # 2 (32 + 28 K) B; per pixel the incoming gradient and log T 16 B.
+    run(False)
+    offsets, prims, early, csr = rz.last_lists
+    tiles = offsets.numel() - 1
+    starts = offsets[:-1]
+    idx = torch.arange(prims.numel(), device="cuda")
+    tile_of = torch.repeat_interleave(torch.arange(tiles, device="cuda"), (offsets[1:] - starts))
+    live = (idx - starts[tile_of]) < early.long()[tile_of]
+    deg = (csr[1:] - csr[:-1]).long()[prims.long()]
+    n_live = float(live.sum().item())
+    faces_live = float(deg[live].sum().item())
+    n_isect = float(prims.numel())
+    fwd_bytes = n_live * (32 + 8 + 224 + 4) + faces_live * 8 + W * H * 32 + sc["points"].shape[0] * 16 + n_isect * 24
+    bwd_bytes = n_live * (32 + 8 + 224 + 4 + 2 * (32 + 28 * 8)) + faces_live * 12 + W * H * 16
+    achieved = (fwd_bytes + bwd_bytes) / (times[True] * 1e-3) / 1e9
     out = {"adjacency": "cech", "adjacency_edges": int(adj.numel()), "cech_build_s": round(cech_s, 2),
            "fwd_ms": round(times[False], 3), "fwd_bwd_ms": round(times[True], 3),
            "mrays_per_s": round(W * H / (times[True] * 1e-3) / 1e6, 3),
+           "fwd_alg_bytes": fwd_bytes, "bwd_alg_bytes": bwd_bytes,
+           "roofline": {"bound": "hbm", "kernel": "raster fwd+bwd", "achieved": round(achieved, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4)},
            "workload": f"Rasterizer.forward + backward of an L2 loss, same scene, camera and target as the walk"}
     log(f"raster: Cech {adj.numel()} edges in {cech_s:.2f}s; fwd {times[False]:.2f} ms, fwd+bwd {times[True]:.2f} ms")
     return out
@@ -405,7 +560,7 @@ def serve_trace(scenes, make, args, label):
     s_wall = srv.run(s_trace)
     s_stats = latency_stats([r for r in s_trace if not r.rejected])
     # identical images: the served result equals the plain path's
-    same = all(torch.equal(a.color, b.color) for a, b in zip(s_trace[:8], p_trace[:8]) if not a.rejected)
+    same = all(torch.equal(a.color, b.color) for a, b in zip(s_trace, p_trace) if not a.rejected)
     out = {
         "requests": len(trace), "rate_hz": round(rate, 2), "load": args.serve_load,
         "p50_ms": round(s_stats["p50_ms"], 3), "p99_ms": round(s_stats["p99_ms"], 3),
@@ -415,6 +570,7 @@ def serve_trace(scenes, make, args, label):
         "rejected": len([r for r in s_trace if r.rejected]), "status": int(srv.status.item()),
         "tile_capacity": args.serve_capacity, "quantum": args.serve_quantum, "colour": srv.colour,
         "results_equal_plain": bool(same),
+        "results_compared": len([r for r in s_trace if not r.rejected]),
     }
     log(f"{label}: p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms vs plain p50 {out['plain_p50_ms']} "
         f"p99 {out['plain_p99_ms']} ms, rejected {out['rejected']}")

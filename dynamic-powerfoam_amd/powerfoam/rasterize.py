@@ -200,6 +200,8 @@ class Rasterizer:
                                    int(bool(return_point_err)), int(self.loss_accumulation), ctypes.byref(out),
                                    _lib.stream_ptr())
         _lib.check(rc, "pf_raster_forward")
+        # the last forward's tile lists and stop positions (for byte counts)
+        self.last_lists = (offsets, prims, early, off)
         grad_ctx.rasterizer = self
         grad_ctx.camera = camera
         grad_ctx.state = dict(cam_dev=cam_dev, spheres=spheres, nsigmas=nsigmas, sites=sites, rgb=rgb, height=height,
