@@ -386,6 +386,9 @@ def c3_bench(args):
     f, b = step(want_bytes=True)
     torch.cuda.synchronize()
     fwd_bytes, bwd_bytes = float(f["alg_bytes"].sum().item()), float(b["alg_bytes"].sum().item())
+    cnt = rt.record_counts(f["record"], B, W, H).double()
+    segs = dict(cap=cap, mean=round(cnt.mean().item(), 2), max=int(cnt.max().item()),
+                replayed_rays=int((cnt > cap).sum().item()))
     fwd_ms, bwd_ms = float(np.median(fw)), float(np.median(bw))
     dom_bytes, dom_ms, dom = (fwd_bytes, fwd_ms, "walk_fwd") if fwd_ms >= bwd_ms else (bwd_bytes, bwd_ms, "walk_bwd")
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
@@ -394,10 +397,11 @@ def c3_bench(args):
                        f"+ grouped bwd + SH bwd, L2 loss",
            "ms_per_step": round(dt * 1e3, 3), "mrays_per_s": round(rays / dt / 1e6, 2), "walk_fwd_ms": round(fwd_ms, 3),
            "walk_bwd_ms": round(bwd_ms, 3), "walk_fwd_alg_bytes": fwd_bytes, "walk_bwd_alg_bytes": bwd_bytes,
-           "record_cap": cap, "status": int(f["status"].item()) | int(b["status"].item()),
+           "record": segs, "status": int(f["status"].item()) | int(b["status"].item()),
            "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4)}}
-    log(f"16 cameras: step {dt * 1e3:.2f} ms = {out['mrays_per_s']} Mrays/s; fwd {fwd_ms:.2f} ms, bwd {bwd_ms:.2f} ms")
+    log(f"16 cameras: step {dt * 1e3:.2f} ms = {out['mrays_per_s']} Mrays/s; fwd {fwd_ms:.2f} ms, bwd {bwd_ms:.2f} ms; "
+        f"record {segs}")
     return out
 
 

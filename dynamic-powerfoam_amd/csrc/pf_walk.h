@@ -883,18 +883,24 @@ struct FaceTerms {
 // gradients go to sink (sphere, [n, sigma], texel sites, colours,
 // heights); neighbour spheres get global atomics, or, with ft, are handed
 // back for the caller to sum across lanes first.
+// The segment's incoming terms: Tg = T_s g (the ray's colour gradient
+// scaled by the transmittance before the segment) and Lnext =
+// g.(C - C_{<=s}) + g_logT, the gradient of the log transmittance after it.
+__device__ __forceinline__ f3 seg_Tg(f3 g, float trans) { return g * trans; }
+__device__ __forceinline__ float seg_Lnext(f3 g, float gL, f3 Cfin, f3 Cs) { return dot(g, Cfin - Cs) + gL; }
+
 template <typename A, typename F, int KT, typename RGB, typename Sink>
 __device__ __forceinline__ void segment_grads_to(const SceneDev<A> &s, const RGB &colors, f3 o, f3 d, const Seg &q,
-                                                 const TexFwdT<KT> &tx, f3 g, float gL, f3 Cfin, const GradDev &G,
+                                                 const TexFwdT<KT> &tx, f3 Tg, float Lnext, const GradDev &G,
                                                  bool exact, const Sink &sink, FaceTerms *ft = nullptr) {
     if (ft) ft->nb[0] = ft->nb[1] = -1;
     const int K = KT > 0 ? KT : s.K;
     const int cell = q.cell, n_adj = q.n_adj;
     const f3 c = q.c;
-    const float r = q.r, sigma = q.sigma, trans = q.trans, dt = q.dt;
-    const float dL_dLnext = dot(g, Cfin - q.rgb) + gL;
-    const float dL_dalpha = dot(g, tx.rgb) * trans;
-    const f3 dL_dcolor = (g * trans) * q.alpha;
+    const float r = q.r, sigma = q.sigma, dt = q.dt;
+    const float dL_dLnext = Lnext;
+    const float dL_dalpha = dot(Tg, tx.rgb);
+    const f3 dL_dcolor = Tg * q.alpha;
     const float dL_ddelta = dL_dLnext + (-dL_dalpha * q.e_dlt);
     const float dL_dsigma = dL_ddelta * (-dt);
     const float dL_dtn = dL_ddelta * sigma;
@@ -967,7 +973,8 @@ __device__ __forceinline__ void segment_grads(const SceneDev<A> &s, const RGB &c
                               G.height + tbase,
                               K,
                               pr};
-    segment_grads_to<A, F, KT>(s, colors, o, d, q, tx, g, gL, Cfin, G, exact, sink);
+    segment_grads_to<A, F, KT>(s, colors, o, d, q, tx, seg_Tg(g, q.trans), seg_Lnext(g, gL, Cfin, q.rgb), G, exact,
+                               sink);
 }
 
 // Backward by checkpoint and replay.  The forward's per-ray colour C and

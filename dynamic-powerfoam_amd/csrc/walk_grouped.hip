@@ -7,8 +7,12 @@
 //   dL/d(delta_s) = g.(C - C_{<=s}) + g_logT - (g.c_s) T_s exp(delta_s)
 // needs only the ray's incoming gradient g, g_logT and final colour C.  So
 // the segments can be differentiated in any order.  This file sorts them by
-// cell (a counting sort on the device: count, scan, scatter) and
-// differentiates them in cell order: each wave takes a contiguous range of
+// cell (a counting sort on the device: count, scan, scatter; the scatter
+// writes each segment as a self-contained payload, its record entry, the
+// ray's slot and the two scalars of the ray it needs, T_s g and
+// g.(C - C_{<=s}) + g_logT, so the gradient pass reads the sorted segments
+// in order and gathers nothing per ray) and differentiates them in cell
+// order: each wave takes a contiguous range of
 // the sorted segments, 64 at a time with every lane busy, so runs of one
 // cell sit in neighbouring lanes.  A segmented scan across the lanes (DPP
 // row shifts and row broadcasts) sums each gradient term over its run, and
@@ -36,7 +40,10 @@ namespace pf {
 constexpr int kGroupRows = 16;                  // LDS rows per wave (runs held at once)
 constexpr int kGroupPitch = 8 + 7 * 8 + 1;      // K = 8: 64 terms, odd pitch
 constexpr int kScanChunk = 4096;                // entries per scan block (256 x 16)
-constexpr int kReplayBlocks = 16;               // blocks replaying rays past the cap
+#ifndef PF_REPLAY_BLOCKS
+#define PF_REPLAY_BLOCKS 256
+#endif
+constexpr int kReplayBlocks = PF_REPLAY_BLOCKS;  // blocks replaying rays past the cap
 constexpr int kGroupBlocksPerCU = PF_GROUP_WAVES;
 
 struct GroupWs {
@@ -45,7 +52,10 @@ struct GroupWs {
     int *blk;         // scan block totals (nblk + 1)
     int *over_count;  // rays past the cap
     int *over_list;   // S: their slots
-    int *sorted;      // S * cap: record entry ids in bucket order
+    // the segments in bucket order, each self-contained (S * cap entries):
+    int4 *seg;        // the record entry {cell, t_near, t_far, face ids}
+    float4 *coef;     // {T_s g, g.(C - C_<=s) + g_logT}
+    int *slot;        // the ray's slot
 };
 
 struct GroupParams {
@@ -141,12 +151,25 @@ template <bool FILL> __global__ __launch_bounds__(256) void grp_bucket_kernel(Gr
     const int N = (int)p.s.num_cells;
     const int lane = __lane_id();
     const uint64_t lt = (1ull << lane) - 1;
+    // the ray's incoming gradient and final colour, read once per ray
+    f3 g = mk(0, 0, 0), Cfin = mk(0, 0, 0);
+    float gL = 0.0f;
+    if (FILL && count > 0) {
+        const int64_t ray = sr.ray;
+        if (p.g_color) g = mk(p.g_color[3 * ray], p.g_color[3 * ray + 1], p.g_color[3 * ray + 2]);
+        if (p.g_log_t) gL = p.g_log_t[ray];
+        Cfin = mk(p.ck_color[3 * ray], p.ck_color[3 * ray + 1], p.ck_color[3 * ray + 2]);
+        // per ray: incoming gradients, checkpoint colour, segment count
+        if (p.alg_bytes) atomicAdd((unsigned long long *)(p.alg_bytes + ray), 12ull + 12 + 4 + (p.g_log_t ? 4 : 0));
+    }
     for (int k = 0; __any(k < count); ++k) {
         int bk = -1;
         int64_t e = 0;
+        int4 rq = make_int4(0, 0, 0, 0);
         if (k < count) {
             e = rec_index(k, slot, q.S);
-            const int cell = p.rec_segs[e].x;
+            rq = p.rec_segs[e];
+            const int cell = rq.x;
             if ((unsigned)cell < (unsigned)N) bk = bucket_of(q, cell, sr.b);
             else if (!FILL && p.status) atomicOr(p.status, PF_STATUS_BAD_RECORD);
         }
@@ -168,7 +191,15 @@ template <bool FILL> __global__ __launch_bounds__(256) void grp_bucket_kernel(Gr
             int base = 0;
             if (lead_here) base = atomicAdd(q.ws.cursor + bk, n);
             base = __shfl(base, leader, 64);
-            if (bk >= 0) q.ws.sorted[base + rank] = (int)e;
+            if (bk >= 0) {
+                const float4 pt = q.part[e];
+                const f3 Tg = seg_Tg(g, pt.x);
+                const float Ln = seg_Lnext(g, gL, Cfin, mk(pt.y, pt.z, pt.w));
+                const int pos = base + rank;
+                q.ws.seg[pos] = rq;
+                q.ws.coef[pos] = make_float4(Tg.x, Tg.y, Tg.z, Ln);
+                q.ws.slot[pos] = (int)slot;
+            }
         } else if (lead_here) {
             atomicAdd(q.ws.bucket_off + bk, n);
         }
@@ -396,10 +427,9 @@ __device__ __forceinline__ void grouped_range(const GroupParams &q, int64_t lo, 
     for (int64_t i0 = lo; i0 < hi; i0 += 64) {
         const int64_t i = i0 + lane;
         const bool act = i < hi;
-        const int e = act ? q.ws.sorted[i] : 0;
-        const int4 rq = act ? p.rec_segs[e] : make_int4(0, 0, 0, 0);
-        const float4 pt = act ? q.part[e] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        const SlotRay sr = slot_ray32(p, ((uint32_t)e >> 3) % (uint32_t)q.S);
+        const int4 rq = act ? q.ws.seg[i] : make_int4(0, 0, 0, 0);
+        const float4 cf = act ? q.ws.coef[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const SlotRay sr = slot_ray32(p, act ? (uint32_t)q.ws.slot[i] : 0u);
         const int cell = act ? rq.x : 0;
         const int key = act ? bucket_of(q, cell, sr.b) : -2;
         // runs of equal buckets; lane 0 continues the carried run if equal
@@ -425,11 +455,6 @@ __device__ __forceinline__ void grouped_range(const GroupParams &q, int64_t lo, 
         const f3 d = ray_dir(cam, sr.pix_i, sr.pix_j);
         const f3 o = cam.eye;
         const int64_t ray = act ? sr.ray : 0;
-        const f3 g = act && p.g_color ? mk(p.g_color[3 * ray], p.g_color[3 * ray + 1], p.g_color[3 * ray + 2])
-                                      : mk(0, 0, 0);
-        const float gL = act && p.g_log_t ? p.g_log_t[ray] : 0.0f;
-        const f3 Cfin = act ? mk(p.ck_color[3 * ray], p.ck_color[3 * ray + 1], p.ck_color[3 * ray + 2])
-                            : mk(0, 0, 0);
         const MemRGB<A> colors{reinterpret_cast<const A *>(p.s.texel_rgb) + (int64_t)sr.b * p.s.texel_rgb_cam_stride};
         float tn = __int_as_float(rq.y), tf = __int_as_float(rq.z);
         int tn_id = (int)(int16_t)(rq.w & 0xffff);
@@ -460,16 +485,17 @@ __device__ __forceinline__ void grouped_range(const GroupParams &q, int64_t lo, 
                              G.rgb + (int64_t)sr.b * rgb_stride + 3 * tbase, G.height + tbase};
         FaceTerms ft;
         segment_grads_to<A, F, 8>(s, colors, o, d,
-                                  Seg{cell, c, nrm, r, sigma, tn0, dt, e_dlt, alpha, pt.x, mk(pt.y, pt.z, pt.w),
-                                      tn_id, tf_id, tn0_id, tf0_id, e0, n_adj, tbase},
-                                  tx, g, gL, Cfin, G, p.exact_texel, sink, &ft);
+                                  Seg{cell, c, nrm, r, sigma, tn0, dt, e_dlt, alpha, 0.0f, mk(0, 0, 0), tn_id,
+                                      tf_id, tn0_id, tf0_id, e0, n_adj, tbase},
+                                  tx, mk(cf.x, cf.y, cf.z), cf.w, G, p.exact_texel, sink, &ft);
         add_face_terms(act ? ft.nb[0] : -1, ft.dc[0], ft.dr[0], G.spheres);
         add_face_terms(act ? ft.nb[1] : -1, ft.dc[1], ft.dr[1], G.spheres);
         if (act && p.alg_bytes) {
             constexpr uint64_t kVisit = WalkBytes<A, F>::visit, kTexel = WalkBytes<A, F>::texel;
-            // record entry and partials, its sort (two cell reads, the id
-            // written and read), the cell, texels and gradient rows
-            const uint64_t bytes = kSegRecordBytes + kSegPartBytes + 16 + kVisit + 8 * kTexel +
+            // record entry and partials (read once), the sorted payload
+            // (written and read: 36 B each way), the cell, texels and
+            // gradient rows
+            const uint64_t bytes = kSegRecordBytes + kSegPartBytes + 2 * 36 + kVisit + 8 * kTexel +
                                    segment_grad_bytes(8, tn0_id, tf0_id, n_adj);
             atomicAdd((unsigned long long *)(p.alg_bytes + ray), (unsigned long long)bytes);
         }
@@ -544,8 +570,11 @@ __device__ __forceinline__ void replay_over(const GroupParams &q, int64_t first,
     }
 }
 
-// Blocks [0, kReplayBlocks) replay the rays past the cap (dispatched first,
-// their long walks overlap the rest); the others split the sorted segments
+// Blocks [0, kReplayBlocks) replay the rays past the cap, one per lane
+// (dispatched first, their long walks overlap the rest; a block with no such
+// ray returns at once, so many blocks cost nothing when few rays pass the cap
+// and keep a low cap, where a tenth of the rays may pass it, to one pass of
+// replays instead of dozens); the others split the sorted segments
 // into equal contiguous ranges, one per wave, numbered XCD-aware so that one
 // XCD's waves take neighbouring ranges (the same cells' rows).
 template <typename A, typename F>
@@ -575,7 +604,7 @@ using namespace pf;
 static inline int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
 struct GroupLayout {
-    int64_t off_bucket, off_cursor, off_blk, off_over_count, off_over_list, off_sorted, total;
+    int64_t off_bucket, off_cursor, off_blk, off_over_count, off_over_list, off_seg, off_coef, off_slot, total;
     int64_t nb, nblk;
 };
 
@@ -600,7 +629,11 @@ static bool group_layout(int64_t num_cells, int32_t num_cams, int32_t width, int
     o = align256(o + 4);
     L.off_over_list = o;
     o = align256(o + 4 * S);
-    L.off_sorted = o;
+    L.off_seg = o;
+    o = align256(o + 16 * S * cap);
+    L.off_coef = o;
+    o = align256(o + 16 * S * cap);
+    L.off_slot = o;
     o = align256(o + 4 * S * cap);
     L.total = o;
     return true;
@@ -747,7 +780,9 @@ static int grouped_backward(const pf_scene *scene, const pf_camera *cams, int32_
     q.ws.blk = reinterpret_cast<int *>(ws + L.off_blk);
     q.ws.over_count = reinterpret_cast<int *>(ws + L.off_over_count);
     q.ws.over_list = reinterpret_cast<int *>(ws + L.off_over_list);
-    q.ws.sorted = reinterpret_cast<int *>(ws + L.off_sorted);
+    q.ws.seg = reinterpret_cast<int4 *>(ws + L.off_seg);
+    q.ws.coef = reinterpret_cast<float4 *>(ws + L.off_coef);
+    q.ws.slot = reinterpret_cast<int *>(ws + L.off_slot);
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     q.grouped_blocks = cus * kGroupBlocksPerCU;

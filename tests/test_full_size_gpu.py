@@ -118,6 +118,7 @@ def _per_camera_oracle_backward(sc, adj, off, faces, rgb, words, color, gc, gl, 
     for b in range(B):
         cs = _cpu_scene(sc, adj, off, faces, rgb[b])
         cb = orc.walk_backward(cs, words[b:b + 1], color[b:b + 1], gc[b:b + 1], gl[b:b + 1], exact=exact)
+        print(f"oracle backward: camera {b + 1} of {B}", flush=True)
         for k in ("spheres", "nsigmas", "texel_sites", "texel_height"):
             out[k] = out[k] + cb[k]
         out["texel_rgb"].append(cb["texel_rgb"])
@@ -251,6 +252,7 @@ def test_262k_cells_16_cameras_finite_differences():
             continue
         a.append(float(an[name][idx]))
         fd.append(central)
+        print(f"finite differences: {len(a)} of 64 scalars ({tries} drawn)", flush=True)
     assert len(a) == 64, f"only {len(a)} of {tries} scalars were away from a jump or bend"
     assert _rel(np.array(a), np.array(fd)) < 1e-3
 

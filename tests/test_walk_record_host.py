@@ -46,12 +46,13 @@ def test_counts_image_follows_the_tile_lane_map():
 def test_record_cap_fits_budget():
     import types
 
-    tracer = rt.RayTracer(types.SimpleNamespace(walk_record_cap=64, walk_record_budget=1 << 30), "cpu")
+    tracer = rt.RayTracer(types.SimpleNamespace(walk_record_cap=64, walk_record_budget=2 << 30), "cpu")
     cap = tracer.record_cap_for(1, 1920, 1080)
-    assert 8 <= cap < 64 and cap % 8 == 0 and rt.record_bytes(1, 1920, 1080, cap) <= 1 << 30
+    assert 8 <= cap < 64 and cap % 8 == 0 and rt.record_bytes(1, 1920, 1080, cap) <= 2 << 30
     assert tracer.record_cap_for(64, 1920, 1080) == 0
     assert rt.RayTracer(types.SimpleNamespace(walk_record_cap=0), "cpu").record_cap_for(1, 64, 64) == 0
-    # the default: 128 segments per ray, 4.2 GB for one 1080p frame
+    # the default: 128 segments per ray, 8.5 GB of record and 9.6 GB of
+    # grouped workspace for one 1080p frame
     assert rt.RayTracer(None, "cpu").record_cap_for(1, 1920, 1080) == rt.DEFAULT_RECORD_CAP == 128
 
 
