@@ -7,6 +7,7 @@ rasterizer.  Kernels: ``csrc/sv.hip`` (``pf_sv_forward`` /
 ``pf_sv_backward``).
 """
 import math
+import types
 
 import torch
 
@@ -125,3 +126,29 @@ class SphericalVoronoi:
                                             _lib.stream_ptr())
             _lib.check(rc, "pf_sv_backward")
         return None, None, None, g_axes, g_vals, g_temps
+
+
+class _SavedCtx:
+    """What SphericalVoronoi._forward saves, kept for a backward run outside
+    autograd (the serving path colours a request once and differentiates it
+    after its last tile)."""
+
+    def save_for_backward(self, *tensors):
+        self.saved_tensors = tensors
+
+
+def sv_prepass(points, camera, att_sites, att_values, att_temps, keep_for_backward=False):
+    """texel_rgb (N*K, 3) of the lobes for one camera, outside autograd.
+    With keep_for_backward, returns (texel_rgb, ctx) and ctx feeds
+    sv_prepass_backward."""
+    dtype = "half" if att_sites.dtype == torch.float16 else "float"
+    sv = SphericalVoronoi(types.SimpleNamespace(sv_dof=att_sites.shape[0]), points.device, dtype)
+    ctx = _SavedCtx() if keep_for_backward else None
+    out = sv._forward(ctx, points, camera, att_sites, att_values, att_temps)
+    return (out, ctx) if keep_for_backward else out
+
+
+def sv_prepass_backward(ctx, grad_values):
+    """(d axes, d values, d temperatures) of the lobes from dL/dtexel_rgb
+    (N*K, 3), for a pre-pass run with keep_for_backward."""
+    return ctx.spherical_voronoi._backward(ctx, grad_values)[3:]

@@ -23,7 +23,11 @@ Colour: with ``colour="prepass"`` (default) each admitted request of an
 SH-coloured scene gets its own texel colours from one SH pre-pass for its
 camera, which the step kernel then reads like the plain path does; with
 ``colour="walk"`` the walk evaluates the SH at every shaded segment and no
-per-request colour buffer is needed (less memory, more arithmetic).
+per-request colour buffer is needed (less memory, more arithmetic).  A
+scene coloured by spherical-Voronoi lobes (a trained reference checkpoint,
+``ResidentScene.from_powerfoam``) always gets the per-request pre-pass
+(``SphericalVoronoi.forward``, color_fn.py:188-246, as PowerfoamScene
+colours its texels before a render, scene.py:331-335).
 
 Admission: a request reserves its output buffers (its colour buffer, and
 gradient buffers for forward+backward) against ``memory_budget``; it
@@ -42,7 +46,9 @@ import torch
 
 from . import _lib
 from .camera import cameras_to_device, pack_cameras
-from .raytrace import DEFAULT_RECORD_CAP, WalkScene, launch_walk_backward, launch_walk_forward, start_cells_device
+from .color_fn import sv_prepass, sv_prepass_backward
+from .raytrace import (DEFAULT_RECORD_CAP, WalkScene, launch_walk_backward, launch_walk_forward, start_cells_device,
+                       walk_faces)
 from .shading import sh_backward, sh_forward
 
 TILE = 8
@@ -64,6 +70,7 @@ class RenderRequest:
     color: torch.Tensor = None
     log_t: torch.Tensor = None
     grads: dict = None
+    colour_grads: object = None  # PlainRenderer: the colour pre-pass's gradients
     admitted: float = None
     done: float = None
     rejected: bool = False
@@ -72,6 +79,7 @@ class RenderRequest:
     _submitted: float = 0.0
     _cam_dev: torch.Tensor = None
     _texel_rgb: torch.Tensor = None
+    _sv_ctx: object = None
     _events: list = field(default_factory=list)
 
     @property
@@ -89,13 +97,16 @@ class RenderRequest:
 
 class ResidentScene:
     """Device arrays of one scene (f32 attributes, packed f16 walk faces, K = 8)
-    plus optional degree-3 SH coefficients (turned into texel colours by a
-    per-request pre-pass, or evaluated inside the walk)."""
+    plus its colour: fixed texel_rgb, degree-3 SH coefficients (turned into
+    texel colours by a per-request pre-pass, or evaluated inside the walk),
+    or spherical-Voronoi lobes ``sv = (axes (D, N*K, 3), values (D, N*K, 3),
+    temperatures (D, N*K))`` in the lobe-major layout of
+    PowerfoamScene.get_att_sv (scene.py:290-312), coloured per request."""
 
     def __init__(self, points, radii, density, normals, texel_sites, texel_height, adjacency, adjacency_offsets,
-                 faces, sh=None, texel_rgb=None):
-        if sh is None and texel_rgb is None:
-            raise ValueError("a scene needs SH coefficients or texel_rgb")
+                 faces, sh=None, texel_rgb=None, sv=None):
+        if sh is None and texel_rgb is None and sv is None:
+            raise ValueError("a scene needs SH coefficients, spherical-Voronoi lobes or texel_rgb")
         K = texel_sites.shape[1]
         if K != 8 or faces.dtype not in (torch.float16, torch.int32):
             raise ValueError("served scenes use K = 8 texels and fp16 (or packed) walk faces")
@@ -113,20 +124,69 @@ class ResidentScene:
             raise ValueError("SH coefficients must be (N, 16, 3) (degree 3)")
         self.num_cells = points.shape[0]
         self.K = K
+        self.sv = None
+        if sv is not None:
+            axes, vals, temps = (t.detach().float().contiguous() for t in sv)
+            D, nk = axes.shape[0], self.num_cells * K
+            if axes.shape != (D, nk, 3) or vals.shape != (D, nk, 3) or temps.shape != (D, nk):
+                raise ValueError("spherical-Voronoi lobes must be axes (D, N*K, 3), values (D, N*K, 3) and "
+                                 "temperatures (D, N*K)")
+            self.sv = (axes, vals, temps)
+            self.texel_sites_flat = self.walk.texel_sites.reshape(-1, 3)
         s = _lib.PfServeScene()
         s.scene = self.walk.c
         s.sh = _lib.ptr(self.sh)
         s.sh_degree = 3 if self.sh is not None else -1
         self.c = s
 
+    @classmethod
+    def from_powerfoam(cls, scene, power_adjacency=None):
+        """A resident copy of a PowerfoamScene, e.g. one read from a reference
+        checkpoint with load_pt (scene.py:801-821): activated attributes in
+        f32, world texel sites and heights (scene.py:324-337), the lobes for
+        the per-request colour pre-pass, a power-diagram adjacency (built on
+        the GPU unless given as (adjacency, offsets)) and its fp16 walk
+        faces.  Served requests then equal PowerfoamScene.render_walk on the
+        same adjacency."""
+        from .power import build_power_adjacency_gpu
+
+        with torch.no_grad():
+            f32 = lambda t: t.detach().float().contiguous()
+            pts, rad = f32(scene.points), f32(scene.get_radii())
+            density, normals = f32(scene.get_density()), f32(scene.get_normals())
+            sites, heights = (f32(t) for t in scene.world_texels())
+            sv = tuple(f32(t) for t in scene.get_att_sv())
+        if power_adjacency is None:
+            adj, off = build_power_adjacency_gpu(pts, rad)
+        else:
+            adj, off = (t.to(pts.device).int().contiguous() for t in power_adjacency)
+        faces = walk_faces(pts, rad, adj, off)
+        return cls(pts, rad, density, normals, sites, heights, adj, off, faces, sv=sv)
+
     def nbytes(self):
         w = self.walk
         tensors = [w.spheres, w.nsigmas, w.texel_sites, w.texel_height, w.adjacency, w.offsets, w.faces]
         if self.sh is not None:
             tensors.append(self.sh)
+        elif self.sv is not None:
+            tensors.extend(self.sv)
         else:
             tensors.append(w.texel_rgb)
         return sum(t.numel() * t.element_size() for t in tensors)
+
+    def colour_prepass(self, camera, cams_dev, keep_for_backward=False):
+        """This camera's texel colours (N, K, 3) from the SH or lobes, or None
+        for a scene with fixed texel_rgb.  For lobes with keep_for_backward,
+        returns (colours, context of sv_prepass_backward)."""
+        if self.sv is not None:
+            out = sv_prepass(self.texel_sites_flat, camera, *self.sv, keep_for_backward=keep_for_backward)
+            rgb, ctx = out if keep_for_backward else (out, None)
+            rgb = rgb.view(self.num_cells, self.K, 3)
+            return (rgb, ctx) if keep_for_backward else rgb
+        if self.sh is not None:
+            rgb = sh_forward(self.sh, 3, self.walk.texel_sites, cams_dev)[0]
+            return (rgb, None) if keep_for_backward else rgb
+        return (None, None) if keep_for_backward else None
 
     def grad_bytes(self):
         n, K = self.num_cells, self.K
@@ -212,13 +272,15 @@ class RenderServer:
 
     # ---------------------------------------------------------- requests --
     def _prepass(self, r):
-        return self.colour == "prepass" and self.scenes[r.scene].sh is not None
+        sc = self.scenes[r.scene]
+        return sc.sv is not None or (self.colour == "prepass" and sc.sh is not None)
 
     def _request_bytes(self, r):
         b = r.width * r.height * 16
         if self._prepass(r):
             sc = self.scenes[r.scene]
-            b += sc.num_cells * sc.K * 12
+            # colours, and the lobes' weight sums kept for their backward
+            b += sc.num_cells * sc.K * (16 if sc.sv is not None else 12)
         if r.mode == "forward_backward":
             b += self.scenes[r.scene].grad_bytes() + r.width * r.height * 12
         return b
@@ -288,7 +350,9 @@ class RenderServer:
             r._cam_dev = cameras_to_device([r.camera], None, dev)
             start_cells_device(sc.points, sc.radii, r._cam_dev)
             if self._prepass(r):  # this camera's texel colours, once, ahead of its tiles on the same stream
-                r._texel_rgb = sh_forward(sc.sh, 3, sc.walk.texel_sites, r._cam_dev)[0]
+                r._texel_rgb, r._sv_ctx = sc.colour_prepass(r.camera, r._cam_dev, keep_for_backward=True)
+                if r.mode != "forward_backward":
+                    r._sv_ctx = None
             r.admitted = now
             r._next_tile = 0
             self.active.append(r)
@@ -451,10 +515,17 @@ class RenderServer:
                     self.on_finish(r)
 
     def finish_gradients(self, r):
-        """dL/dSH of a finished forward+backward request (SH backward of its
-        texel-colour gradient)."""
+        """Colour-parameter gradients of a finished forward+backward request:
+        dL/dSH (SH backward of its texel-colour gradient), or for a lobe
+        scene (d axes, d values, d temperatures) (SphericalVoronoi's
+        backward, color_fn.py:10-24); None for fixed texel_rgb."""
         sc = self.scenes[r.scene]
-        if sc.sh is None or r.grads is None:
+        if r.grads is None:
+            return None
+        if sc.sv is not None:
+            ctx, r._sv_ctx = r._sv_ctx, None
+            return sv_prepass_backward(ctx, r.grads["texel_rgb"].reshape(-1, 3))
+        if sc.sh is None:
             return None
         return sh_backward(sc.sh, 3, sc.walk.texel_sites, r._cam_dev, r.grads["texel_rgb"][None])
 
@@ -485,8 +556,9 @@ class _SceneView:
 
 
 class PlainRenderer:
-    """Per-request drop-in path: SH pre-pass + walk forward (+ backward and
-    SH backward), one request at a time, as a user of RayTracer would."""
+    """Per-request drop-in path: colour pre-pass (SH or lobes) + walk forward
+    (+ backward and colour backward), one request at a time, as a user of
+    RayTracer would."""
 
     def __init__(self, scenes, transmittance_threshold=1e-3):
         self.scenes = scenes
@@ -499,8 +571,8 @@ class PlainRenderer:
         cam = cameras_to_device([r.camera], None, dev)
         start_cells_device(sc.points, sc.radii, cam)
         w = sc.walk
-        if sc.sh is not None:
-            rgb = sh_forward(sc.sh, 3, w.texel_sites, cam)
+        rgb, sv_ctx = sc.colour_prepass(r.camera, cam, keep_for_backward=True)
+        if rgb is not None:
             c = _lib.PfScene.from_buffer_copy(w.c)
             c.texel_rgb = rgb.data_ptr()
             scene = _SceneView(c, w.num_cells, w.K, rgb)
@@ -516,8 +588,10 @@ class PlainRenderer:
             gc = 2.0 * (out["color"] - tgt[None])
             r.grads = launch_walk_backward(scene, cam, r.width, r.height, out["color"], out["log_t"], gc, None,
                                            self.thr, record=out["record"])
-            if sc.sh is not None:
-                sh_backward(sc.sh, 3, w.texel_sites, cam, r.grads["texel_rgb"][None])
+            if sc.sv is not None:
+                r.colour_grads = sv_prepass_backward(sv_ctx, r.grads["texel_rgb"].reshape(-1, 3))
+            elif sc.sh is not None:
+                r.colour_grads = sh_backward(sc.sh, 3, w.texel_sites, cam, r.grads["texel_rgb"][None])
         return r
 
     def run(self, requests):
