@@ -72,9 +72,15 @@ def main():
                     help="also time the 16-camera differentiation configuration (262,144 cells, 800x800, 0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (reference kernel for HBM byte counters)")
+    ap.add_argument("--serve-sweep", action="store_true",
+                    help="only run both serving traces at loads 0.3/0.6/0.9 and at saturation, server and plain "
+                         "path, and print that as one JSON object")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.serve_sweep:
+        serve_sweep(args)
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -514,20 +520,15 @@ def build_served_scene(n, seed):
                          t(sc["texel_height"])[perm], adj, off, faces, sh=t(sc["sh"])[perm])
 
 
-def serve_trace(scenes, make, args, label):
-    """Serve one Poisson trace with the continuous-batching server and with
-    the plain per-request path (one request at a time, FIFO), at
-    serve_load x the plain path's measured capacity.  Returns the stats."""
-    import copy
-
-    import numpy as np
+def plain_service(scenes, make):
+    """Warm the plain path on a mix of every kind of request (the
+    allocator's cache) and measure its mean service time per request (s).
+    Returns (warm-up requests, mean service time)."""
     import torch
 
-    from powerfoam.serving import PlainRenderer, RenderServer, latency_stats
+    from powerfoam.serving import PlainRenderer
 
-    rng = np.random.default_rng(2000 + args.seed)
     plain = PlainRenderer(scenes)
-    # warm-up on a mix of every kind (the allocator's cache, graph capture)
     warm = make(32)
     for r in warm:
         plain.render(r)
@@ -537,14 +538,41 @@ def serve_trace(scenes, make, args, label):
     for r in probe:
         plain.render(r)
         torch.cuda.synchronize()
-    mean_service = (time.perf_counter() - t0) / len(probe)
-    rate = args.serve_load / mean_service
+    return warm, (time.perf_counter() - t0) / len(probe)
+
+
+def serve_trace(scenes, make, args, label, load=None, calibration=None):
+    """Serve one trace with the continuous-batching server and with the
+    plain per-request path (one request at a time, FIFO).  Poisson arrivals
+    at load x the plain path's measured capacity (load defaults to
+    --serve-load); load "saturation" submits every request at once (closed
+    loop: throughput at saturation, nothing rejected for waiting).
+    calibration: (warm-up requests, plain mean service) from plain_service.
+    Returns the stats."""
+    import copy
+
+    import numpy as np
+    import torch
+
+    from powerfoam.serving import PlainRenderer, RenderServer, latency_stats
+
+    load = args.serve_load if load is None else load
+    rng = np.random.default_rng(2000 + args.seed)
+    plain = PlainRenderer(scenes)
+    warm, mean_service = calibration if calibration is not None else plain_service(scenes, make)
     trace = make(args.serve_requests)
-    arr = np.cumsum(rng.exponential(1.0 / rate, len(trace)))
-    for r, a in zip(trace, arr):
-        r.arrival = float(a)
+    if load == "saturation":
+        rate = None
+        for r in trace:
+            r.arrival = 0.0
+    else:
+        rate = load / mean_service
+        arr = np.cumsum(rng.exponential(1.0 / rate, len(trace)))
+        for r, a in zip(trace, arr):
+            r.arrival = float(a)
     rays = sum(r.width * r.height for r in trace)
-    log(f"{label}: {len(trace)} requests, rate {rate:.1f}/s (plain mean service {mean_service * 1e3:.2f} ms)")
+    log(f"{label}: {len(trace)} requests, load {load}, rate {rate if rate is None else round(rate, 1)}/s "
+        f"(plain mean service {mean_service * 1e3:.2f} ms)")
     def consume(r):  # a client takes the gradients and lets them go
         r.grads = None
 
@@ -554,7 +582,7 @@ def serve_trace(scenes, make, args, label):
     p_stats = latency_stats(p_trace)
     srv = RenderServer("cuda", memory_budget=64 << 30, tile_capacity=args.serve_capacity, quantum=args.serve_quantum,
                        oldest_share=args.serve_oldest_share, streams=args.serve_streams,
-                       record_cap=args.serve_record_cap)
+                       record_cap=args.serve_record_cap, max_wait=1e9 if load == "saturation" else 2.0)
     for name, scene in scenes.items():
         srv.add_scene(name, scene)
     srv.on_finish = consume
@@ -566,7 +594,7 @@ def serve_trace(scenes, make, args, label):
     # identical images: the served result equals the plain path's
     same = all(torch.equal(a.color, b.color) for a, b in zip(s_trace, p_trace) if not a.rejected)
     out = {
-        "requests": len(trace), "rate_hz": round(rate, 2), "load": args.serve_load,
+        "requests": len(trace), "rate_hz": None if rate is None else round(rate, 2), "load": load,
         "p50_ms": round(s_stats["p50_ms"], 3), "p99_ms": round(s_stats["p99_ms"], 3),
         "mrays_per_s": round(rays / s_wall / 1e6, 2),
         "plain_p50_ms": round(p_stats["p50_ms"], 3), "plain_p99_ms": round(p_stats["p99_ms"], 3),
@@ -575,6 +603,8 @@ def serve_trace(scenes, make, args, label):
         "tile_capacity": args.serve_capacity, "quantum": args.serve_quantum, "colour": srv.colour,
         "results_equal_plain": bool(same),
         "results_compared": len([r for r in s_trace if not r.rejected]),
+        "wall_s": round(s_wall, 3), "plain_wall_s": round(p_wall, 3),
+        "plain_mean_service_ms": round(mean_service * 1e3, 3),
     }
     log(f"{label}: p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms vs plain p50 {out['plain_p50_ms']} "
         f"p99 {out['plain_p99_ms']} ms, rejected {out['rejected']}")
@@ -584,10 +614,10 @@ def serve_trace(scenes, make, args, label):
 RESOLUTIONS = [(256, 256), (512, 512), (1024, 1024), (1920, 1080)]
 
 
-def serve_bench(args):
-    """Serving form, one resident scene of --serve-cells cells (3M): a
-    Poisson stream of forward requests, resolutions uniform over {256^2,
-    512^2, 1024^2, 1920x1080}, cameras on the radius-3 sphere."""
+def single_scene_trace(args):
+    """One resident scene of --serve-cells cells (3M), forward requests,
+    resolutions uniform over {256^2, 512^2, 1024^2, 1920x1080}, cameras on
+    the radius-3 sphere.  Returns (scenes, request maker, workload)."""
     import numpy as np
 
     from powerfoam import workloads
@@ -605,17 +635,17 @@ def serve_bench(args):
             reqs.append(RenderRequest(scene="s", camera=workloads.sphere_cameras(rng, 1, w, h)[0], rid=k))
         return reqs
 
-    out = serve_trace(scenes, make, args, "serving")
-    out["workload"] = (f"{args.serve_cells}-cell GMM scene resident, SH3 colour pre-pass per request, forward "
-                       "requests, resolutions uniform over 256^2/512^2/1024^2/1920x1080, Poisson arrivals")
-    return out
+    workload = (f"{args.serve_cells}-cell GMM scene resident, SH3 colour pre-pass per request, forward "
+                "requests, resolutions uniform over 256^2/512^2/1024^2/1920x1080, Poisson arrivals")
+    return scenes, make, workload
 
 
-def serve_multi_bench(args):
-    """Serving form, several resident scenes under memory admission: cell
-    counts log-uniform in [1e5, 2e6], each request picks a scene uniformly,
-    resolutions as above, a quarter of the requests forward+backward against
-    a random target image, a 64 GB budget."""
+def multi_scene_trace(args):
+    """Several resident scenes under memory admission: cell counts
+    log-uniform in [1e5, 2e6], each request picks a scene uniformly,
+    resolutions as above, a quarter of the requests forward+backward
+    against a random target image, a 64 GB budget.  Returns (scenes,
+    request maker, workload, cell counts)."""
     import numpy as np
     import torch
 
@@ -639,12 +669,54 @@ def serve_multi_bench(args):
                                       mode="forward_backward" if bwd else "forward", target=tgt))
         return reqs
 
-    out = serve_trace(scenes, make, args, "serving (multi-scene)")
-    out["scene_cells"] = [int(n) for n in sizes]
-    out["workload"] = (f"{len(sizes)} GMM scenes resident (cells log-uniform in [1e5, 2e6]), a scene picked "
-                       "uniformly per request, 25% forward+backward requests against random targets, "
-                       "resolutions uniform over 256^2/512^2/1024^2/1920x1080, Poisson arrivals, 64 GB budget")
+    workload = (f"{len(sizes)} GMM scenes resident (cells log-uniform in [1e5, 2e6]), a scene picked "
+                "uniformly per request, 25% forward+backward requests against random targets, "
+                "resolutions uniform over 256^2/512^2/1024^2/1920x1080, Poisson arrivals, 64 GB budget")
+    return scenes, make, workload, [int(n) for n in sizes]
+
+
+def serve_bench(args):
+    """Serving form, one resident scene (single_scene_trace) at --serve-load."""
+    scenes, make, workload = single_scene_trace(args)
+    out = serve_trace(scenes, make, args, "serving")
+    out["workload"] = workload
     return out
+
+
+def serve_multi_bench(args):
+    """Serving form, several scenes under admission (multi_scene_trace) at
+    --serve-load."""
+    scenes, make, workload, sizes = multi_scene_trace(args)
+    out = serve_trace(scenes, make, args, "serving (multi-scene)")
+    out["scene_cells"] = sizes
+    out["workload"] = workload
+    return out
+
+
+SWEEP_LOADS = (0.3, 0.6, 0.9, "saturation")
+
+
+def serve_sweep(args):
+    """Both serving traces at offered loads 0.3, 0.6 and 0.9 of the plain
+    path's capacity and at saturation (every request submitted at once):
+    p50/p99 latency and throughput of the server and the plain path.
+    Prints one JSON object (not the benchmark line)."""
+    import torch
+
+    out = {"loads": [str(x) for x in SWEEP_LOADS]}
+    for name, builder in (("single_scene", single_scene_trace), ("multi_scene", multi_scene_trace)):
+        if name == "multi_scene" and args.serve_scenes <= 0:
+            continue
+        built = builder(args)
+        scenes, make, workload = built[:3]
+        cal = plain_service(scenes, make)
+        rows = [serve_trace(scenes, make, args, f"{name} load {ld}", load=ld, calibration=cal) for ld in SWEEP_LOADS]
+        out[name] = {"workload": workload, "rows": rows}
+        if len(built) > 3:
+            out[name]["scene_cells"] = built[3]
+        del scenes, make, built, cal
+        torch.cuda.empty_cache()
+    print(json.dumps(out), flush=True)
 
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "walk_gmm1m_1080p_hbm_traffic.md")
