@@ -46,6 +46,18 @@ constexpr int kScanChunk = 4096;                // entries per scan block (256 x
 constexpr int kReplayBlocks = PF_REPLAY_BLOCKS;  // blocks replaying rays past the cap
 constexpr int kGroupBlocksPerCU = PF_GROUP_WAVES;
 
+// One sorted segment, self-contained: its record entry, the two scalars of
+// its ray that its gradients need, the ray's direction and camera, its ray
+// and slot.  64 bytes, 64-byte aligned: the scatter writes whole 64-byte
+// sectors (a partial sector costs HBM a read-modify-write).
+struct alignas(64) GroupEntry {
+    int4 seg;     // {cell, t_near bits, t_far bits, face ids}
+    float4 coef;  // {T_s g, g.(C - C_<=s) + g_logT}
+    float4 dirb;  // {ray direction, camera (int bits)}
+    int4 aux;     // {ray, slot, 0, 0}
+};
+static_assert(sizeof(GroupEntry) == 64, "sorted segment entry");
+
 struct GroupWs {
     int *bucket_off;  // nb + 1: counts, then exclusive offsets (bucket_off[nb] = total)
     int *cursor;      // nb: scatter cursors
@@ -53,9 +65,7 @@ struct GroupWs {
     int *over_count;  // rays past the cap
     int *over_list;   // S: their slots
     // the segments in bucket order, each self-contained (S * cap entries):
-    int4 *seg;        // the record entry {cell, t_near, t_far, face ids}
-    float4 *coef;     // {T_s g, g.(C - C_<=s) + g_logT}
-    int *slot;        // the ray's slot
+    GroupEntry *ent;  // the sorted segments
 };
 
 struct GroupParams {
@@ -151,57 +161,76 @@ template <bool FILL> __global__ __launch_bounds__(256) void grp_bucket_kernel(Gr
     const int N = (int)p.s.num_cells;
     const int lane = __lane_id();
     const uint64_t lt = (1ull << lane) - 1;
-    // the ray's incoming gradient and final colour, read once per ray
-    f3 g = mk(0, 0, 0), Cfin = mk(0, 0, 0);
+    // the ray's incoming gradient, final colour and direction, once per ray
+    f3 g = mk(0, 0, 0), Cfin = mk(0, 0, 0), dir = mk(0, 0, 0);
     float gL = 0.0f;
     if (FILL && count > 0) {
         const int64_t ray = sr.ray;
+        dir = ray_dir(load_cam(p.cams + 12 * sr.b), sr.pix_i, sr.pix_j);
         if (p.g_color) g = mk(p.g_color[3 * ray], p.g_color[3 * ray + 1], p.g_color[3 * ray + 2]);
         if (p.g_log_t) gL = p.g_log_t[ray];
         Cfin = mk(p.ck_color[3 * ray], p.ck_color[3 * ray + 1], p.ck_color[3 * ray + 2]);
         // per ray: incoming gradients, checkpoint colour, segment count
         if (p.alg_bytes) atomicAdd((unsigned long long *)(p.alg_bytes + ray), 12ull + 12 + 4 + (p.g_log_t ? 4 : 0));
     }
-    for (int k = 0; __any(k < count); ++k) {
-        int bk = -1;
-        int64_t e = 0;
-        int4 rq = make_int4(0, 0, 0, 0);
-        if (k < count) {
-            e = rec_index(k, slot, q.S);
-            rq = p.rec_segs[e];
-            const int cell = rq.x;
-            if ((unsigned)cell < (unsigned)N) bk = bucket_of(q, cell, sr.b);
-            else if (!FILL && p.status) atomicOr(p.status, PF_STATUS_BAD_RECORD);
-        }
-        uint64_t pend = __ballot(bk >= 0);
-        int leader = 0, rank = 0, n = 0;
-        while (pend) {
-            const int lead = __ffsll((unsigned long long)pend) - 1;
-            const int b = __builtin_amdgcn_readlane(bk, lead);
-            const uint64_t mem = __ballot(bk == b);
-            pend &= ~mem;
-            if (bk == b) {
-                leader = lead;
-                rank = __builtin_popcountll(mem & lt);
-                n = __builtin_popcountll(mem);
+    // A ray's segments come eight to a 128-byte line (rec_index): each lane
+    // reads its line's eight entries (and partials) at once.  Read one at a
+    // time, the wave's 64 lines were evicted from L2 between the reads and
+    // fetched again for every segment.
+    for (int c = 0; __any(kRecChunk * c < count); ++c) {
+        int4 rc[kRecChunk];
+        float4 pc[kRecChunk];
+        if (kRecChunk * c < count) {
+            const int64_t e0 = rec_index(kRecChunk * c, slot, q.S);
+#pragma unroll
+            for (int j = 0; j < kRecChunk; ++j) rc[j] = p.rec_segs[e0 + j];
+            if (FILL) {
+#pragma unroll
+                for (int j = 0; j < kRecChunk; ++j) pc[j] = q.part[e0 + j];
             }
         }
-        const bool lead_here = bk >= 0 && leader == lane;
-        if (FILL) {
-            int base = 0;
-            if (lead_here) base = atomicAdd(q.ws.cursor + bk, n);
-            base = __shfl(base, leader, 64);
-            if (bk >= 0) {
-                const float4 pt = q.part[e];
-                const f3 Tg = seg_Tg(g, pt.x);
-                const float Ln = seg_Lnext(g, gL, Cfin, mk(pt.y, pt.z, pt.w));
-                const int pos = base + rank;
-                q.ws.seg[pos] = rq;
-                q.ws.coef[pos] = make_float4(Tg.x, Tg.y, Tg.z, Ln);
-                q.ws.slot[pos] = (int)slot;
+#pragma unroll
+        for (int j = 0; j < kRecChunk; ++j) {
+            const int k = kRecChunk * c + j;
+            int bk = -1;
+            int4 rq = make_int4(0, 0, 0, 0);
+            if (k < count) {
+                rq = rc[j];
+                const int cell = rq.x;
+                if ((unsigned)cell < (unsigned)N) bk = bucket_of(q, cell, sr.b);
+                else if (!FILL && p.status) atomicOr(p.status, PF_STATUS_BAD_RECORD);
             }
-        } else if (lead_here) {
-            atomicAdd(q.ws.bucket_off + bk, n);
+            uint64_t pend = __ballot(bk >= 0);
+            int leader = 0, rank = 0, n = 0;
+            while (pend) {
+                const int lead = __ffsll((unsigned long long)pend) - 1;
+                const int b = __builtin_amdgcn_readlane(bk, lead);
+                const uint64_t mem = __ballot(bk == b);
+                pend &= ~mem;
+                if (bk == b) {
+                    leader = lead;
+                    rank = __builtin_popcountll(mem & lt);
+                    n = __builtin_popcountll(mem);
+                }
+            }
+            const bool lead_here = bk >= 0 && leader == lane;
+            if (FILL) {
+                int base = 0;
+                if (lead_here) base = atomicAdd(q.ws.cursor + bk, n);
+                base = __shfl(base, leader, 64);
+                if (bk >= 0) {
+                    const float4 pt = pc[j];
+                    const f3 Tg = seg_Tg(g, pt.x);
+                    const float Ln = seg_Lnext(g, gL, Cfin, mk(pt.y, pt.z, pt.w));
+                    GroupEntry *en = q.ws.ent + (base + rank);
+                    en->seg = rq;
+                    en->coef = make_float4(Tg.x, Tg.y, Tg.z, Ln);
+                    en->dirb = make_float4(dir.x, dir.y, dir.z, __int_as_float(sr.b));
+                    en->aux = make_int4((int)sr.ray, (int)slot, 0, 0);
+                }
+            } else if (lead_here) {
+                atomicAdd(q.ws.bucket_off + bk, n);
+            }
         }
     }
 }
@@ -427,11 +456,13 @@ __device__ __forceinline__ void grouped_range(const GroupParams &q, int64_t lo, 
     for (int64_t i0 = lo; i0 < hi; i0 += 64) {
         const int64_t i = i0 + lane;
         const bool act = i < hi;
-        const int4 rq = act ? q.ws.seg[i] : make_int4(0, 0, 0, 0);
-        const float4 cf = act ? q.ws.coef[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        const SlotRay sr = slot_ray32(p, act ? (uint32_t)q.ws.slot[i] : 0u);
+        const GroupEntry *en = q.ws.ent + (act ? i : 0);
+        const int4 rq = act ? en->seg : make_int4(0, 0, 0, 0);
+        const float4 cf = act ? en->coef : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float4 db = act ? en->dirb : make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+        const int ray_b = __float_as_int(db.w);
         const int cell = act ? rq.x : 0;
-        const int key = act ? bucket_of(q, cell, sr.b) : -2;
+        const int key = act ? bucket_of(q, cell, ray_b) : -2;
         // runs of equal buckets; lane 0 continues the carried run if equal
         const int prev = __shfl_up(key, 1, 64);
         const bool head = lane == 0 ? key != carry : key != prev;
@@ -451,11 +482,10 @@ __device__ __forceinline__ void grouped_range(const GroupParams &q, int64_t lo, 
                            -(int)(lane >= 32 && 31 >= rs)};
 
         // ---- the segment: recorded t values, partials, texel pass
-        const Cam cam = load_cam(p.cams + 12 * sr.b);
-        const f3 d = ray_dir(cam, sr.pix_i, sr.pix_j);
-        const f3 o = cam.eye;
-        const int64_t ray = act ? sr.ray : 0;
-        const MemRGB<A> colors{reinterpret_cast<const A *>(p.s.texel_rgb) + (int64_t)sr.b * p.s.texel_rgb_cam_stride};
+        const float *eye = p.cams + 12 * ray_b;
+        const f3 d = mk(db.x, db.y, db.z);
+        const f3 o = mk(eye[0], eye[1], eye[2]);
+        const MemRGB<A> colors{reinterpret_cast<const A *>(p.s.texel_rgb) + (int64_t)ray_b * p.s.texel_rgb_cam_stride};
         float tn = __int_as_float(rq.y), tf = __int_as_float(rq.z);
         int tn_id = (int)(int16_t)(rq.w & 0xffff);
         int tf_id = rq.w >> 16;
@@ -482,7 +512,7 @@ __device__ __forceinline__ void grouped_range(const GroupParams &q, int64_t lo, 
         const float alpha = 1.0f - e_dlt;
         const GroupSink sink{scan, act, end, run < kGroupRows ? rows + run * kGroupPitch : nullptr,
                              G.spheres + 4 * (int64_t)cell, G.nsigmas + 4 * (int64_t)cell, G.sites + 3 * tbase,
-                             G.rgb + (int64_t)sr.b * rgb_stride + 3 * tbase, G.height + tbase};
+                             G.rgb + (int64_t)ray_b * rgb_stride + 3 * tbase, G.height + tbase};
         FaceTerms ft;
         segment_grads_to<A, F, 8>(s, colors, o, d,
                                   Seg{cell, c, nrm, r, sigma, tn0, dt, e_dlt, alpha, 0.0f, mk(0, 0, 0), tn_id,
@@ -492,12 +522,11 @@ __device__ __forceinline__ void grouped_range(const GroupParams &q, int64_t lo, 
         add_face_terms(act ? ft.nb[1] : -1, ft.dc[1], ft.dr[1], G.spheres);
         if (act && p.alg_bytes) {
             constexpr uint64_t kVisit = WalkBytes<A, F>::visit, kTexel = WalkBytes<A, F>::texel;
-            // record entry and partials (read once), the sorted payload
-            // (written and read: 36 B each way), the cell, texels and
-            // gradient rows
-            const uint64_t bytes = kSegRecordBytes + kSegPartBytes + 2 * 36 + kVisit + 8 * kTexel +
-                                   segment_grad_bytes(8, tn0_id, tf0_id, n_adj);
-            atomicAdd((unsigned long long *)(p.alg_bytes + ray), (unsigned long long)bytes);
+            // record entry and partials (read once), the sorted entry
+            // (written and read), the cell, texels and gradient rows
+            const uint64_t bytes = kSegRecordBytes + kSegPartBytes + 2 * sizeof(GroupEntry) + kVisit +
+                                   8 * kTexel + segment_grad_bytes(8, tn0_id, tf0_id, n_adj);
+            atomicAdd((unsigned long long *)(p.alg_bytes + en->aux.x), (unsigned long long)bytes);
         }
         __builtin_amdgcn_wave_barrier();
 
@@ -517,7 +546,7 @@ __device__ __forceinline__ void grouped_range(const GroupParams &q, int64_t lo, 
             const int lead = __ffsll((unsigned long long)hm) - 1;
             hm &= hm - 1;
             const int rc = __builtin_amdgcn_readlane(cell, lead);
-            const int rb = __builtin_amdgcn_readlane(sr.b, lead);
+            const int rb = __builtin_amdgcn_readlane(ray_b, lead);
             const int rk = __builtin_amdgcn_readlane(key, lead);
             if (hm) {
                 if (r_idx < kGroupRows) flush_row(rows + r_idx * kGroupPitch, G, rc, rb, rgb_stride);
@@ -604,7 +633,7 @@ using namespace pf;
 static inline int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
 struct GroupLayout {
-    int64_t off_bucket, off_cursor, off_blk, off_over_count, off_over_list, off_seg, off_coef, off_slot, total;
+    int64_t off_bucket, off_cursor, off_blk, off_over_count, off_over_list, off_ent, total;
     int64_t nb, nblk;
 };
 
@@ -629,12 +658,8 @@ static bool group_layout(int64_t num_cells, int32_t num_cams, int32_t width, int
     o = align256(o + 4);
     L.off_over_list = o;
     o = align256(o + 4 * S);
-    L.off_seg = o;
-    o = align256(o + 16 * S * cap);
-    L.off_coef = o;
-    o = align256(o + 16 * S * cap);
-    L.off_slot = o;
-    o = align256(o + 4 * S * cap);
+    L.off_ent = o;
+    o = align256(o + (int64_t)sizeof(GroupEntry) * S * cap);
     L.total = o;
     return true;
 }
@@ -780,9 +805,7 @@ static int grouped_backward(const pf_scene *scene, const pf_camera *cams, int32_
     q.ws.blk = reinterpret_cast<int *>(ws + L.off_blk);
     q.ws.over_count = reinterpret_cast<int *>(ws + L.off_over_count);
     q.ws.over_list = reinterpret_cast<int *>(ws + L.off_over_list);
-    q.ws.seg = reinterpret_cast<int4 *>(ws + L.off_seg);
-    q.ws.coef = reinterpret_cast<float4 *>(ws + L.off_coef);
-    q.ws.slot = reinterpret_cast<int *>(ws + L.off_slot);
+    q.ws.ent = reinterpret_cast<GroupEntry *>(ws + L.off_ent);
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     q.grouped_blocks = cus * kGroupBlocksPerCU;

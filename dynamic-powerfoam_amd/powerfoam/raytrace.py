@@ -20,10 +20,10 @@ TILE_SIZE = TILE_WIDTH * TILE_WIDTH
 DEFAULT_MAX_STEPS = 1 << 16
 # contributing segments per ray the differentiable forward records for its
 # backward, and the most device memory one record and its grouped-backward
-# workspace may take (68 B per segment slot; 16 cameras of 800x800 keep a cap
-# of 56 under it, where a cap of 24 leaves 2% of the rays to replay)
+# workspace may take (96 B per segment slot; 16 cameras of 800x800 keep a cap
+# of 64 under it, where a cap of 24 leaves 2% of the rays to replay)
 DEFAULT_RECORD_CAP = 128
-DEFAULT_RECORD_BUDGET = 40 << 30
+DEFAULT_RECORD_BUDGET = 64 << 30
 
 
 def _contig(t, dtype=None):
@@ -431,13 +431,13 @@ class RayTracer:
 
     def record_cap_for(self, num_cams, width, height):
         """Segments per ray the differentiable forward records: record_cap,
-        lowered to fit record_budget bytes (68 per segment slot: the
-        segment and its partials, 32, and its sorted payload in the grouped
-        backward's workspace, 36); 0 (replay) below 8."""
+        lowered to fit record_budget bytes (96 per segment slot: the
+        segment and its partials, 32, and its sorted entry in the grouped
+        backward's workspace, 64); 0 (replay) below 8."""
         if self.record_cap <= 0:
             return 0
         fit = (self.record_budget - 8 * record_slots(num_cams, width, height)) // \
-            (68 * record_slots(num_cams, width, height))
+            (96 * record_slots(num_cams, width, height))
         cap = min(self.record_cap, int(fit)) // RECORD_CHUNK * RECORD_CHUNK
         return cap if cap >= 8 else 0
 

@@ -51,7 +51,7 @@ def test_record_cap_fits_budget():
     assert 8 <= cap < 64 and cap % 8 == 0 and rt.record_bytes(1, 1920, 1080, cap) <= 2 << 30
     assert tracer.record_cap_for(64, 1920, 1080) == 0
     assert rt.RayTracer(types.SimpleNamespace(walk_record_cap=0), "cpu").record_cap_for(1, 64, 64) == 0
-    # the default: 128 segments per ray, 8.5 GB of record and 9.6 GB of
+    # the default: 128 segments per ray, 8.5 GB of record and 17 GB of
     # grouped workspace for one 1080p frame
     assert rt.RayTracer(None, "cpu").record_cap_for(1, 1920, 1080) == rt.DEFAULT_RECORD_CAP == 128
 
