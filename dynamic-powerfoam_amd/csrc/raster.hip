@@ -956,12 +956,13 @@ extern "C" int pf_raster_faces(const float *spheres, int64_t num_cells, const in
     return hip_ok();
 }
 
-// At most 2^24 8x8 tiles (1.07 G pixels) per image: the per-tile sort runs
-// one 256-thread workgroup per tile, under 2^32 work-items per dispatch.
+// Fewer than 2^24 8x8 tiles (1.07 G pixels) per image: the per-tile sort
+// runs one 256-thread workgroup per tile, under 2^32 work-items per
+// dispatch.
 constexpr int64_t kMaxRasterTiles = int64_t(1) << 24;
 
 static bool raster_grid_fits(int32_t W, int32_t H) {
-    return (int64_t)((H + 7) / 8) * ((W + 7) / 8) <= kMaxRasterTiles;
+    return (int64_t)((H + 7) / 8) * ((W + 7) / 8) < kMaxRasterTiles;
 }
 
 extern "C" int pf_raster_bin_count(const float *spheres, int64_t num_cells, const pf_camera *cam,
@@ -993,7 +994,7 @@ extern "C" int pf_raster_bin_write(const float *spheres, int64_t num_cells, cons
                                    uint64_t *scratch, int32_t *tile_prims, uint8_t *visible_mask,
                                    int64_t num_isect, void *stream) {
     if (!spheres || num_cells <= 0 || !cam || !tile_offsets || !write_counter || num_tiles <= 0) return PF_ERR_ARG;
-    if (num_tiles > kMaxRasterTiles) return PF_ERR_ARG;
+    if (num_tiles >= kMaxRasterTiles) return PF_ERR_ARG;
     if (num_isect > 0 && (!keys || !scratch || !tile_prims)) return PF_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(bin_write_kernel, dim3((unsigned)((num_cells + 255) / 256)), dim3(256), 0, st,

@@ -200,8 +200,9 @@ extern "C" int pf_serve_step(const pf_serve_scene *scenes, const pf_serve_reques
                              int64_t tile_capacity, float transmittance_threshold, int32_t max_steps, int32_t flags,
                              int32_t *status, const pf_walk_record *record, void *stream) {
     if (!scenes || !requests || !desc || tile_capacity <= 0 || max_steps <= 0) return PF_ERR_ARG;
-    // one 256-thread workgroup per four tiles, under 2^32 work-items
-    if (tile_capacity > (int64_t(1) << 26)) return PF_ERR_ARG;
+    // one 256-thread workgroup per four tiles (64 work-items per tile), under
+    // 2^32 work-items: fewer than 2^26 tiles
+    if (tile_capacity >= (int64_t(1) << 26)) return PF_ERR_ARG;
     if (flags & ~PF_SERVE_SH_IN_WALK) return PF_ERR_ARG;
     if (record && (!record->segments || record->cap <= 0 || record->cap % kRecChunk)) return PF_ERR_ARG;
     const unsigned blocks = (unsigned)((tile_capacity + 3) / 4);

@@ -348,13 +348,13 @@ static int check_scene(const pf_scene *s) {
     return PF_OK;
 }
 
-// The recorded backward launches two 256-thread workgroups per four tiles,
-// and a dispatch holds fewer than 2^32 work-items: at most 2^25 tiles of
-// 8x8 rays (2.1 G rays) per call.
+// The recorded backward launches two 256-thread workgroups per four tiles
+// (128 work-items per tile), and a dispatch holds fewer than 2^32
+// work-items: fewer than 2^25 tiles of 8x8 rays (2.1 G rays) per call.
 constexpr int64_t kMaxWalkTiles = int64_t(1) << 25;
 
 static bool grid_fits(int32_t B, int32_t W, int32_t H) {
-    return (int64_t)B * ((H + 7) / 8) * ((W + 7) / 8) <= kMaxWalkTiles;
+    return (int64_t)B * ((H + 7) / 8) * ((W + 7) / 8) < kMaxWalkTiles;
 }
 
 static WalkParams make_params(const pf_scene *scene, const pf_camera *cams, int32_t B, int32_t W, int32_t H,

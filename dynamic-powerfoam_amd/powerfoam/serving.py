@@ -199,8 +199,8 @@ class RenderServer:
                  oldest_share=0.0, streams=3, record_cap=128):
         if colour not in ("prepass", "walk"):
             raise ValueError("colour must be 'prepass' or 'walk'")
-        if not 0 < int(tile_capacity) <= 1 << 26 or int(quantum) <= 0 or int(streams) < 1:
-            raise ValueError("tile_capacity must be in [1, 2^26], quantum positive and streams at least 1")
+        if not 0 < int(tile_capacity) < 1 << 26 or int(quantum) <= 0 or int(streams) < 1:
+            raise ValueError("tile_capacity must be in [1, 2^26), quantum positive and streams at least 1")
         self.colour = colour
         self.device = device
         self.budget = int(memory_budget)

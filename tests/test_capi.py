@@ -76,7 +76,9 @@ def test_walk_arguments_rejected_before_any_launch():
     assert fwd(1, 0, 8) == _lib.PF_ERR_ARG
     assert fwd(1, 8, -1) == _lib.PF_ERR_ARG
     assert fwd(0, 8, 8) == _lib.PF_ERR_ARG
-    # 2^25 tiles is the largest grid: 4096 cameras of 8192 x 8192 rays exceed it
+    # fewer than 2^25 tiles per grid (128 work-items each, under 2^32): 32
+    # cameras of 8192 x 8192 rays are exactly 2^25 tiles, 4096 far more
+    assert fwd(32, 8192, 8192) == _lib.PF_ERR_ARG
     assert fwd(4096, 8192, 8192) == _lib.PF_ERR_ARG
     bad_dtype = _lib.PfScene.from_buffer_copy(scene)
     bad_dtype.face_dtype = 7
@@ -93,8 +95,13 @@ def test_walk_arguments_rejected_before_any_launch():
     assert lib.pf_walk_backward_recorded(ctypes.byref(scene), fake, 1, 8, 8, 1e-4, 64, fake, fake, fake, None,
                                          ctypes.byref(rec), ctypes.byref(grads), 0, None, None,
                                          None) == _lib.PF_ERR_ARG
-    assert lib.pf_walk_backward(ctypes.byref(scene), fake, 4096, 8192, 8192, 1e-4, 64, fake, fake, fake, None,
-                                ctypes.byref(grads), 0, None, None, None) == _lib.PF_ERR_ARG
+    for B in (32, 4096):
+        assert lib.pf_walk_backward(ctypes.byref(scene), fake, B, 8192, 8192, 1e-4, 64, fake, fake, fake, None,
+                                    ctypes.byref(grads), 0, None, None, None) == _lib.PF_ERR_ARG
+        assert lib.pf_walk_backward_recorded(ctypes.byref(scene), fake, B, 8192, 8192, 1e-4, 64, fake, fake, fake,
+                                             None, ctypes.byref(_lib.PfWalkRecord(segments=fake, counts=fake,
+                                                                                  cap=16)),
+                                             ctypes.byref(grads), 0, None, None, None) == _lib.PF_ERR_ARG
     assert lib.pf_walk_record_slots(2, 9, 17) == 2 * 3 * 2 * 64
     assert lib.pf_walk_record_slots(0, 9, 17) == 0
     # the grouped backward: partials required, K = 8, a large enough workspace
@@ -127,13 +134,16 @@ def test_raster_and_serving_sizes_rejected_before_any_launch():
     out = _lib.PfRasterOut(color=fake, log_t=fake, normal_distance=fake, normal=fake, contrib=fake)
     raster = lambda W, H: lib.pf_raster_forward(ctypes.byref(scene), fake, W, H, fake, fake, fake, 1e-4, None, 0,
                                                 None, 0, 0, ctypes.byref(out), None)
-    # 2^24 tiles of 8x8 is the largest raster image
+    # fewer than 2^24 tiles of 8x8 per raster image: 65536 x 16384 is exactly 2^24
+    assert raster(65536, 16384) == _lib.PF_ERR_ARG
     assert raster(65536, 16385) == _lib.PF_ERR_ARG
     assert raster(0, 8) == _lib.PF_ERR_ARG
-    assert lib.pf_raster_bin_write(fake, 10, fake, (1 << 24) + 1, fake, fake, fake, fake, fake, None, 10,
-                                   None) == _lib.PF_ERR_ARG
+    for tiles in (1 << 24, (1 << 24) + 1):
+        assert lib.pf_raster_bin_write(fake, 10, fake, tiles, fake, fake, fake, fake, fake, None, 10,
+                                       None) == _lib.PF_ERR_ARG
     step = lambda cap, flags=0, rec=None: lib.pf_serve_step(fake, fake, fake, cap, 1e-4, 64, flags, None, rec, None)
     assert step(0) == _lib.PF_ERR_ARG
+    assert step(1 << 26) == _lib.PF_ERR_ARG  # 64 work-items per tile: fewer than 2^26 tiles
     assert step((1 << 26) + 1) == _lib.PF_ERR_ARG
     assert step(64, flags=0x10) == _lib.PF_ERR_ARG
     rec = _lib.PfWalkRecord(segments=fake, counts=fake, cap=100)

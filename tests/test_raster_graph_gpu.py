@@ -3,7 +3,11 @@ whole raster forward (bin count, bounded scan, bin write + per-tile sort,
 raster) runs from device memory, is captured in a HIP graph, and every
 replay (with eager work in between) equals the eager result and the
 unbounded path; a capacity below the need raises PF_STATUS_LIST_CAP and
-keeps every access inside the buffers."""
+keeps every access inside the buffers.  The forward, the L2 loss's incoming
+gradient and the backward (rasterize.py:1150-1516) captured in one graph
+give, on every replay, the eager pass's image bit for bit and its
+gradients, which equal autograd through Rasterizer.forward (gradient sums
+are atomics: 1e-5 relative L2)."""
 import ctypes
 import types
 
@@ -77,6 +81,31 @@ class Pass:
                                          P(self.prims), P(self.early), 1e-3, None, 0, None, 0, 0,
                                          ctypes.byref(out), st), "forward")
 
+    def backward_buffers(self, target):
+        n, K = self.spheres.shape[0], self.sites.shape[1]
+        self.target = target.contiguous()
+        self.g_color = torch.zeros(self.H, self.W, 3, device=DEV)
+        self.grads = dict(spheres=torch.zeros(n, 4, device=DEV), nsigmas=torch.zeros(n, 4, device=DEV),
+                          texel_sites=torch.zeros(n, K, 3, device=DEV), texel_rgb=torch.zeros(n, K, 3, device=DEV),
+                          texel_height=torch.zeros(n, K, device=DEV))
+
+    def run_backward(self):
+        """dL/dC = 2 (C - target), then the raster backward into zeroed
+        gradient arrays; no allocation, no host read-back."""
+        lib, st, P = _lib.load(), _lib.stream_ptr(), _lib.ptr
+        torch.sub(self.color, self.target, out=self.g_color)
+        self.g_color.mul_(2.0)
+        for x in self.grads.values():
+            x.zero_()
+        gin = _lib.PfRasterGradIn()
+        gin.color = P(self.g_color)
+        gs = _lib.PfGrads()
+        for k, v in self.grads.items():
+            setattr(gs, k, P(v))
+        _lib.check(lib.pf_raster_backward(ctypes.byref(self.scene), P(self.cam), self.W, self.H, P(self.offsets),
+                                          P(self.prims), P(self.early), None, 0, P(self.log_t), ctypes.byref(gin),
+                                          ctypes.byref(gs), st), "backward")
+
 
 @pytest.fixture(scope="module")
 def setup(scene4k):
@@ -145,3 +174,57 @@ def test_rasterizer_list_capacity_option(setup):
                                         False)
     assert torch.equal(out[0].detach(), ref[0])
     assert torch.isfinite(ps["points"].grad).all()
+
+
+def _rel(a, b):
+    return float(torch.linalg.norm((a - b).flatten()) / max(float(torch.linalg.norm(b.flatten())), 1e-30))
+
+
+def test_forward_and_backward_captured_in_one_graph(setup):
+    sc, adj, off, cam = setup
+    target = torch.from_numpy(np.random.default_rng(4).uniform(0, 1, (cam.height, cam.width, 3))
+                              .astype(np.float32)).to(DEV)
+    p = Pass(sc, adj, off, cam, capacity=1 << 20)
+    p.backward_buffers(target)
+    p.run()
+    p.run_backward()
+    torch.cuda.synchronize()
+    assert int(p.status.item()) == 0
+    eager_color = p.color.clone()
+    eager = {k: v.clone() for k, v in p.grads.items()}
+    # the eager pass against autograd through the drop-in rasterizer
+    ps = {k: t(sc[k]).requires_grad_(True) for k in ("points", "radii", "density", "normals", "texel_sites",
+                                                     "texel_rgb", "texel_height")}
+    out = Rasterizer(ARGS, DEV).forward(cam, None, ps["points"], ps["radii"], ps["density"], ps["normals"],
+                                        ps["texel_sites"], ps["texel_rgb"], ps["texel_height"], t(adj), t(off),
+                                        None, False)
+    ((out[0] - target) ** 2).sum().backward()
+    assert torch.equal(out[0].detach(), eager_color)
+    fin = lambda x: torch.where(torch.isfinite(x), x, torch.zeros_like(x))
+    want = dict(points=eager["spheres"][:, :3], radii=eager["spheres"][:, 3], density=eager["nsigmas"][:, 3],
+                normals=eager["nsigmas"][:, :3], texel_sites=eager["texel_sites"], texel_rgb=eager["texel_rgb"],
+                texel_height=eager["texel_height"])
+    for k, v in want.items():
+        assert float(ps[k].grad.abs().max()) > 0, k
+        assert _rel(fin(v), ps[k].grad) <= 1e-5, k
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        p.run()  # warm-up on the capture stream
+        p.run_backward()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        p.run()
+        p.run_backward()
+    for k in range(3):
+        p.color.fill_(-1.0)  # eager work between replays
+        for v in p.grads.values():
+            v.fill_(7.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(p.color, eager_color), k
+        assert int(p.status.item()) == 0
+        for name, v in p.grads.items():
+            assert _rel(v, eager[name]) <= 1e-5, (k, name)
