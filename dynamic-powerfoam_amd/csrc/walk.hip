@@ -9,12 +9,18 @@
 #include "pf_hip.h"
 #include "pf_launch.h"
 #include "pf_walk.h"
+#include "pf_walk_deferred.h"
 #include "pf_walk_params.h"
 
 namespace pf {
 
+// K = 8 runs the wave walk with deferred, dense texel shading
+// (pf_walk_deferred.h); other K the per-lane walk.  Same results.  Five
+// waves per SIMD (at most 96 VGPRs; the recording walk needs 101 unbounded
+// and fits without spilling).
 template <typename A, typename F, int KT, bool REC>
-__global__ __launch_bounds__(256) void walk_fwd_kernel(WalkParams p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void walk_fwd_kernel(WalkParams p) {
+    __shared__ WaveShade shade[KT == 8 ? 4 : 1];
     const TileLane tl = tile_lane(p, 4, xcd_block(blockIdx.x, gridDim.x));
     if (!tl.valid) return;
     const Cam cam = load_cam(p.cams + 12 * tl.b);
@@ -23,8 +29,13 @@ __global__ __launch_bounds__(256) void walk_fwd_kernel(WalkParams p) {
     const MemRGB<A> colors{reinterpret_cast<const A *>(p.s.texel_rgb) + (int64_t)tl.b * p.s.texel_rgb_cam_stride};
     int *seq = p.out.seq ? p.out.seq + tl.ray * p.out.seq_cap : nullptr;
     const SegRecord rec{p.rec_segs, tl.slot, p.total_tiles * 64, p.rec_cap, p.rec_part};
-    const RayFwd r = walk_ray_fwd<A, F, KT, MemRGB<A>, REC>(s, colors, cam.eye, d, cam.start, p.thr, p.depth_q,
-                                                             p.max_steps, seq, p.out.seq_cap, rec);
+    RayFwd r;
+    if constexpr (KT == 8)
+        r = walk_wave_fwd<A, F, MemRGB<A>, REC>(s, colors, cam.eye, d, cam.start, p.thr, p.depth_q, p.max_steps, seq,
+                                                p.out.seq_cap, rec, shade[threadIdx.x >> 6]);
+    else
+        r = walk_ray_fwd<A, F, KT, MemRGB<A>, REC>(s, colors, cam.eye, d, cam.start, p.thr, p.depth_q, p.max_steps,
+                                                   seq, p.out.seq_cap, rec);
     if constexpr (REC) p.rec_counts[tl.slot] = r.segs;
     float *co = p.out.color + 3 * tl.ray;
     co[0] = r.rgb.x;
