@@ -9,7 +9,7 @@ static_assert(sizeof(pf_walk_out) == 80, "pf_walk_out layout");
 static_assert(sizeof(pf_walk_record) == 40, "pf_walk_record layout");
 static_assert(sizeof(pf_grads) == 48, "pf_grads layout");
 static_assert(sizeof(pf_serve_scene) == 120, "pf_serve_scene layout");
-static_assert(sizeof(pf_serve_request) == 160, "pf_serve_request layout");
+static_assert(sizeof(pf_serve_request) == 168, "pf_serve_request layout");
 static_assert(sizeof(pf_cech_grid) == 32, "pf_cech_grid layout");
 
 extern "C" int pf_device_count(int32_t *n) {

@@ -126,7 +126,8 @@ class PfServeScene(ctypes.Structure):
 class PfServeRequest(ctypes.Structure):
     _fields_ = [("cam", PfCamera), ("scene", c_int32), ("mode", c_int32), ("tile_begin", c_int32),
                 ("tile_count", c_int32), ("step_tile0", c_int64), ("start_cell", c_void_p), ("color", c_void_p),
-                ("log_t", c_void_p), ("target", c_void_p), ("grads", PfGrads), ("texel_rgb", c_void_p)]
+                ("log_t", c_void_p), ("target", c_void_p), ("grads", PfGrads), ("texel_rgb", c_void_p),
+                ("bucket0", c_int64)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/pf_hip.h
@@ -221,6 +222,12 @@ _SIGS = {
         [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_float, c_int32, c_void_p, c_void_p, c_void_p, c_void_p],
     ),
     "pf_serve_record_slots": (c_int64, [c_int64]),
+    "pf_serve_grouped_workspace_bytes": (c_int64, [c_int64, c_int32, c_int64]),
+    "pf_serve_step_grouped": (
+        c_int32,
+        [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_float, c_int32, c_void_p, ctypes.POINTER(PfWalkRecord),
+         c_int64, c_int32, c_void_p, c_int64, c_void_p],
+    ),
     "pf_serve_step": (
         c_int32,
         [c_void_p, c_void_p, c_void_p, c_int64, c_float, c_int32, c_int32, c_void_p, ctypes.POINTER(PfWalkRecord),

@@ -19,6 +19,15 @@ has a queue of its own.  With two, some runs had both on one queue and the
 mixed trace's p99 doubled; with four, two streams shared a queue in some
 runs (multi-scene p99 67-75 ms against 47-54 ms with three).
 
+Forward+backward requests (``backward="grouped"``, the default): their
+tiles in a step record each ray's contributing segments with the
+transmittance before and colour after each, and the step sorts the
+segments of all its forward+backward requests by (request, cell) on the
+device and differentiates them in that order, summing a cell's terms
+across the wave before one atomic per run, as the plain path's grouped walk
+backward does for one image.  ``backward="recorded"`` differentiates each
+tile from its own record through a per-wave LDS cache of gradient rows.
+
 Colour: with ``colour="prepass"`` (default) each admitted request of an
 SH-coloured scene gets its own texel colours from one SH pre-pass for its
 camera, which the step kernel then reads like the plain path does; with
@@ -196,9 +205,12 @@ class ResidentScene:
 class RenderServer:
     def __init__(self, device="cuda", memory_budget=64 << 30, tile_capacity=16384, quantum=1024, use_graph=True,
                  max_wait=2.0, transmittance_threshold=1e-3, max_steps=1 << 16, colour="prepass", bwd_weight=1,
-                 oldest_share=0.0, streams=3, record_cap=128):
+                 oldest_share=0.0, streams=3, record_cap=128, backward="grouped", bwd_capacity=None,
+                 max_buckets=1 << 24, entries_per_slot=None):
         if colour not in ("prepass", "walk"):
             raise ValueError("colour must be 'prepass' or 'walk'")
+        if backward not in ("grouped", "recorded"):
+            raise ValueError("backward must be 'grouped' or 'recorded'")
         if not 0 < int(tile_capacity) < 1 << 26 or int(quantum) <= 0 or int(streams) < 1:
             raise ValueError("tile_capacity must be in [1, 2^26), quantum positive and streams at least 1")
         self.colour = colour
@@ -246,12 +258,34 @@ class RenderServer:
         self.record_cap = -(-int(record_cap) // 8) * 8 if record_cap > 0 else 0
         self._records = None
         self.record_bytes = 0
+        # backward="grouped": a step's forward+backward tiles (at most
+        # bwd_capacity, the whole capacity by default) record their segments
+        # with partials, and the step differentiates all of them sorted by
+        # (request, cell) (pf_serve_step_grouped); a step's forward+backward
+        # requests number their cells from bucket0, at most max_buckets in
+        # all.  Workspace per table buffer: 32 * record_cap + 4 bytes per
+        # slot (64 per tile of bwd_capacity), 64 bytes per sorted entry
+        # (min(record_cap, 48) per slot) and 8 per bucket: 7.6 GB at the
+        # defaults.  A quarter of the capacity held large forward+backward
+        # requests to many steps (multi-scene p99 of those requests 102 ms
+        # against 46 ms).  It needs colours from memory, so colour="walk"
+        # and record_cap 0 differentiate tile by tile ("recorded").
+        self.backward = backward if (colour == "prepass" and self.record_cap > 0) else "recorded"
+        self.bwd_capacity = int(bwd_capacity) if bwd_capacity else self.capacity
+        if not 0 < self.bwd_capacity <= self.capacity:
+            raise ValueError("bwd_capacity must be in [1, tile_capacity]")
+        self.max_buckets = int(max_buckets)
+        # sorted entries the grouped workspace holds per slot (rays past the
+        # budget are replayed): min(record_cap, 48) by default
+        self.entries_per_slot = entries_per_slot
 
     # ------------------------------------------------------------ scenes --
     def add_scene(self, name, scene: ResidentScene):
         if name in self.scenes:
             # requests and the scene table refer to a resident scene by its index
             raise ValueError(f"scene {name!r} is already resident")
+        if self.backward == "grouped" and scene.num_cells > self.max_buckets:
+            raise ValueError(f"scene {name!r} has more cells ({scene.num_cells}) than max_buckets")
         need = scene.nbytes()
         if self.reserved + need > self.budget:
             raise MemoryError(f"scene {name} ({need} B) does not fit in the memory budget")
@@ -286,15 +320,31 @@ class RenderServer:
         return b
 
     def _ensure_records(self):
-        """The forward+backward tiles' record workspaces, allocated with the
-        first such request and counted against the memory budget; if they do
-        not fit, those tiles replay their walks instead (record_cap 0)."""
+        """The forward+backward tiles' record workspaces (one per table
+        buffer), allocated with the first such request and counted against
+        the memory budget.  Grouped: segments and partials of 64 slots per
+        tile of bwd_capacity, counts and the sort's workspace; recorded:
+        segments of 64 slots per tile of capacity.  If they do not fit,
+        those tiles replay their walks instead (record_cap 0)."""
         if self._records is not None or self.record_cap <= 0:
             return
-        slots = _lib.load().pf_serve_record_slots(self.capacity)
-        need = 16 * self.record_cap * slots * self.nbuf
+        lib = _lib.load()
+        grouped = self.backward == "grouped"
+        if grouped:
+            slots = 64 * self.bwd_capacity
+            ws = lib.pf_serve_grouped_workspace_bytes(self.bwd_capacity, self.record_cap, self.max_buckets)
+            if ws <= 0:
+                raise ValueError("bwd_capacity, record_cap or max_buckets out of range")
+            if self.entries_per_slot is not None:  # a smaller sort budget
+                dflt = min(self.record_cap, 48)
+                ws -= 64 * slots * (dflt - max(min(int(self.entries_per_slot), dflt), 1))
+            need = (32 * self.record_cap * slots + 4 * slots + ws) * self.nbuf
+        else:
+            slots = lib.pf_serve_record_slots(self.capacity)
+            need = 16 * self.record_cap * slots * self.nbuf
         if self.reserved + need > self.budget:
             self.record_cap = 0
+            self.backward = "recorded"
             return
         self.reserved += need
         self._records = []
@@ -303,8 +353,16 @@ class RenderServer:
             rec = _lib.PfWalkRecord()
             rec.segments = segs.data_ptr()
             rec.cap = self.record_cap
-            self._records.append((segs, rec))
-        self.record_bytes = 16 * self.record_cap * slots * self.nbuf
+            keep = [segs]
+            if grouped:
+                part = torch.empty(self.record_cap * slots, 4, dtype=torch.float32, device=self.device)
+                counts = torch.empty(slots, dtype=torch.int32, device=self.device)
+                work = torch.empty(ws, dtype=torch.uint8, device=self.device)
+                rec.partials = part.data_ptr()
+                rec.counts = counts.data_ptr()
+                keep += [part, counts, work]
+            self._records.append((keep, rec))
+        self.record_bytes = need
         self._graphs = None  # the launches now carry the workspaces
 
     def submit(self, r: RenderRequest):
@@ -365,29 +423,53 @@ class RenderServer:
         order, leftover capacity to the oldest requests.  Capacity and quantum
         count forward tiles; a forward+backward tile counts bwd_weight of them
         (it costs that much more), so a step's duration stays bounded whatever
-        the mix and a short request never waits behind a long step."""
+        the mix and a short request never waits behind a long step.  With the
+        grouped backward a step also holds at most bwd_capacity
+        forward+backward tiles, and forward+backward requests whose cells
+        together fit max_buckets."""
         plan = []
         cap = self.capacity
+        grouped = self.backward == "grouped" and self._records is not None
+        bwd_left = self.bwd_capacity if grouped else 1 << 62
+        buckets = 0
+
+        def bwd(r):
+            return getattr(r, "mode", "forward") == "forward_backward"
+
+        def room(r, w):
+            lim = cap // w
+            return min(lim, bwd_left) if bwd(r) else lim
+
         for k, r in enumerate(self.active):
             if cap <= 0:
                 break
             w = self._cost(r)
+            if grouped and bwd(r):
+                n = self.scenes[r.scene].num_cells
+                if bwd_left <= 0 or buckets + n > self.max_buckets:
+                    continue
             quantum = self.quantum
             if k == 0:  # the oldest request may take a larger share
                 quantum = max(quantum, int(self.oldest_share * self.capacity))
-            take = min(r.ntiles - r._next_tile, max(quantum // w, 1), cap // w)
+            take = min(r.ntiles - r._next_tile, max(quantum // w, 1), room(r, w))
             if take > 0:
                 plan.append([r, r._next_tile, take])
                 cap -= take * w
+                if bwd(r):
+                    bwd_left -= take
+                    if grouped:
+                        buckets += self.scenes[r.scene].num_cells
         for p in plan:
             if cap <= 0:
                 break
             r, b, take = p
             w = self._cost(r)
-            more = min(r.ntiles - (b + take), cap // w)
+            more = min(r.ntiles - (b + take), room(r, w))
             if more > 0:
                 p[2] += more
                 cap -= more * w
+                if bwd(r):
+                    bwd_left -= more
         return plan
 
     def _write_table(self, buf, plan):
@@ -398,6 +480,7 @@ class RenderServer:
         arr = (_lib.PfServeRequest * max(n, 1))()
         t0 = 0
         fwd_tiles = 0
+        buckets = 0  # grouped backward: each forward+backward entry's cells from bucket0
         for k, (r, b, take) in enumerate(plan):
             e = arr[k]
             w = pack_cameras([r.camera])[0]
@@ -415,13 +498,15 @@ class RenderServer:
                 e.target = r.target.data_ptr()
                 for key in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
                     setattr(e.grads, key, r.grads[key].data_ptr())
+                e.bucket0 = buckets
+                buckets += self.scenes[r.scene].num_cells
             t0 += take
             if r.mode != "forward_backward":
                 fwd_tiles = t0
         host = self._host_tab[buf]
         ctypes.memmove(host.data_ptr(), ctypes.addressof(arr), ctypes.sizeof(arr))
-        desc = np.array([n, t0, fwd_tiles], dtype=np.int64)
-        ctypes.memmove(host.data_ptr() + self.max_reqs * self._req_bytes, desc.ctypes.data, 24)
+        desc = np.array([n, t0, fwd_tiles, buckets], dtype=np.int64)
+        ctypes.memmove(host.data_ptr() + self.max_reqs * self._req_bytes, desc.ctypes.data, 32)
         return t0
 
     def _launch(self, buf):
@@ -429,8 +514,15 @@ class RenderServer:
         host, dev = self._host_tab[buf], self._dev_tab[buf]
         dev.copy_(host, non_blocking=True)
         rec = ctypes.byref(self._records[buf][1]) if self._records is not None else None
-        rc = lib.pf_serve_step(_lib.ptr(self._scene_tab), dev.data_ptr(),
-                               dev.data_ptr() + self.max_reqs * self._req_bytes, self.capacity, self.thr,
+        desc = dev.data_ptr() + self.max_reqs * self._req_bytes
+        if self.backward == "grouped" and self._records is not None:
+            work = self._records[buf][0][3]
+            rc = lib.pf_serve_step_grouped(_lib.ptr(self._scene_tab), dev.data_ptr(), desc, self.capacity,
+                                           self.bwd_capacity, self.thr, self.max_steps, _lib.ptr(self.status), rec,
+                                           self.max_buckets, 0, work.data_ptr(), work.numel(), _lib.stream_ptr())
+            _lib.check(rc, "pf_serve_step_grouped")
+            return
+        rc = lib.pf_serve_step(_lib.ptr(self._scene_tab), dev.data_ptr(), desc, self.capacity, self.thr,
                                self.max_steps, self._flags, _lib.ptr(self.status), rec, _lib.stream_ptr())
         _lib.check(rc, "pf_serve_step")
 
@@ -443,9 +535,9 @@ class RenderServer:
             s = self._streams[buf]
             s.wait_stream(torch.cuda.current_stream())
             with torch.cuda.stream(s):  # warm up outside capture with an empty table
-                desc = np.zeros(3, np.int64)
+                desc = np.zeros(4, np.int64)
                 ctypes.memmove(self._host_tab[buf].data_ptr() + self.max_reqs * self._req_bytes, desc.ctypes.data,
-                               24)
+                               32)
                 self._launch(buf)
         torch.cuda.synchronize()
         for buf in range(self.nbuf):

@@ -498,6 +498,10 @@ typedef struct pf_serve_request {
     const float *target;
     pf_grads grads;
     const float *texel_rgb;
+    int64_t bucket0;            /* pf_serve_step_grouped, mode 1: the first
+                                   of this entry's num_cells buckets (the
+                                   step's mode-1 entries own disjoint,
+                                   increasing ranges); unused otherwise  */
 } pf_serve_request;
 
 /* One serving step.  scenes, requests and desc are device memory; desc =
@@ -523,6 +527,37 @@ int pf_serve_step(const pf_serve_scene *scenes,
                   int64_t tile_capacity, float transmittance_threshold,
                   int32_t max_steps, int32_t flags, int32_t *status,
                   const pf_walk_record *record, void *stream);
+
+/* One serving step whose forward+backward entries are differentiated
+ * grouped by cell, as pf_walk_backward_grouped does for one walk (no
+ * reference counterpart; the backward it replaces per tile is the
+ * reference's rasterize.py:1371-1516 accumulation).  desc = {num_requests,
+ * step_tiles, forward_tiles, buckets}: mode-1 entries cover at most
+ * bwd_tile_capacity tiles and own bucket ranges [bucket0, bucket0 +
+ * num_cells of their scene) within [0, buckets), buckets <= max_buckets;
+ * a table past either limit skips the mode-1 entries and sets
+ * PF_STATUS_LIST_CAP.  Forward tiles run as in pf_serve_step (colours from
+ * memory); mode-1 entries must read colours from memory too (their own
+ * texel_rgb or the scene's).  record: segments and partials hold
+ * cap * 64 * bwd_tile_capacity entries, counts 64 * bwd_tile_capacity
+ * ints (cap a multiple of 8); workspace: any contents, at least its fixed
+ * parts and one sorted entry; pf_serve_grouped_workspace_bytes gives the
+ * size that holds min(cap, 48) entries per slot.  Rays with more than cap
+ * contributing segments, or past the entries the workspace holds, are
+ * replayed in the same launch.  flags: 0 or PF_BWD_EXACT_TEXEL.  Every launch has a fixed
+ * shape (graph capture).  Steps that may run at the same time need
+ * separate records and workspaces.                                       */
+int64_t pf_serve_grouped_workspace_bytes(int64_t bwd_tile_capacity,
+                                         int32_t record_cap,
+                                         int64_t max_buckets);
+int pf_serve_step_grouped(const pf_serve_scene *scenes,
+                          const pf_serve_request *requests,
+                          const int64_t *desc, int64_t tile_capacity,
+                          int64_t bwd_tile_capacity,
+                          float transmittance_threshold, int32_t max_steps,
+                          int32_t *status, const pf_walk_record *record,
+                          int64_t max_buckets, int32_t flags, void *workspace,
+                          int64_t workspace_bytes, void *stream);
 
 /* ---- power-diagram adjacency ----------------------------------------- */
 

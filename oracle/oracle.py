@@ -49,7 +49,7 @@ def load():
         lib.orc_walk_loss_f64.restype = c_double
         lib.orc_walk_loss_f64.argtypes = [c_void_p] * 6 + [c_int32, c_int64, c_void_p, c_void_p, c_int64,
                                                           c_void_p, c_int32, c_double, c_int32, c_void_p, c_int64,
-                                                          c_void_p, c_double, c_int32]
+                                                          c_void_p, c_double, c_int32, c_int64]
         lib.orc_walk_forward.argtypes = [c_void_p, c_void_p, c_int32, c_float, c_float, c_int32, c_void_p, c_int64,
                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
                                          c_int32]
@@ -200,8 +200,8 @@ def walk_backward(scene, cams, color, grad_color, grad_log_t=None, thr=1e-3, max
 def walk_loss_f64(params, adjacency, offsets, cams, target, thr=1e-3, w_alpha=0.0, rays=None, max_steps=1 << 16,
                   nthreads=None):
     """FP64 forward loss (exact faces) for finite differences.  params: dict
-    of float64 arrays points, radii, nsigmas (N,4), texel_sites, texel_rgb,
-    texel_height."""
+    of float64 arrays points, radii, nsigmas (N,4), texel_sites, texel_rgb
+    (N,K,3), or (B,N,K,3) with each camera's own colours, texel_height."""
     lib = load()
     P = {k: _c(v, np.float64) for k, v in params.items()}
     adjacency, offsets = _c(adjacency, np.int32), _c(offsets, np.int32)
@@ -210,10 +210,15 @@ def walk_loss_f64(params, adjacency, offsets, cams, target, thr=1e-3, w_alpha=0.
     K = P["texel_sites"].shape[1]
     N = P["points"].shape[0]
     r = None if rays is None else _c(rays, np.int64)
+    stride = 0
+    if P["texel_rgb"].ndim == 4:
+        if P["texel_rgb"].shape[0] != cams.shape[0]:
+            raise ValueError("per-camera texel_rgb needs one set of colours per camera")
+        stride = N * K * 3
     return lib.orc_walk_loss_f64(_p(P["points"]), _p(P["radii"]), _p(P["nsigmas"]), _p(P["texel_sites"]),
                                  _p(P["texel_rgb"]), _p(P["texel_height"]), K, N, _p(adjacency), _p(offsets),
                                  adjacency.size, _p(cams), cams.shape[0], thr, max_steps, _p(r),
-                                 0 if r is None else r.size, _p(target), w_alpha, nthreads or num_threads())
+                                 0 if r is None else r.size, _p(target), w_alpha, nthreads or num_threads(), stride)
 
 
 # ------------------------------------------------------------ rasterizer --

@@ -802,12 +802,15 @@ struct RasterWeights {
     const double *g_contrib; // (N,) or NULL
 };
 
+// rgb_cam_stride: camera b's texel colours start at rgb + b * rgb_cam_stride
+// (0: one set for every camera).
 static double loss_f64(const double *points, const double *radii, const double *nsigmas, const double *sites,
                        const double *rgb, const double *height, int32_t K, int64_t N, const int32_t *adj,
                        const int32_t *off, int64_t E, const CamIn *cams, int32_t B, double thr, int32_t max_steps,
                        const int64_t *rays, int64_t n_rays, const float *target, double w_alpha,
-                       const RasterWeights *rw, int32_t nthreads) {
+                       const RasterWeights *rw, int32_t nthreads, int64_t rgb_cam_stride = 0) {
     std::vector<double> sph(4 * N), faces(4 * E), pm(N);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
     for (int64_t i = 0; i < N; ++i) {
         sph[4 * i] = points[3 * i];
         sph[4 * i + 1] = points[3 * i + 1];
@@ -816,6 +819,7 @@ static double loss_f64(const double *points, const double *radii, const double *
         const V3<double> p = mk(points[3 * i], points[3 * i + 1], points[3 * i + 2]);
         pm[i] = 0.5 * (dot(p, p) - radii[i] * radii[i]);
     }
+#pragma omp parallel for schedule(static) num_threads(nthreads)
     for (int64_t i = 0; i < N; ++i)
         for (int e = off[i]; e < off[i + 1]; ++e) {
             const int64_t j = adj[e];
@@ -851,7 +855,9 @@ static double loss_f64(const double *points, const double *radii, const double *
             tm.g_contrib = rw->g_contrib;
             tm.inv_hw = 1.0 / ((double)H * W);
         }
-        const RayOut<double> r = walk_ray(s, fl, c.eye, d, c.start, thr, 0.5, max_steps, (int32_t *)nullptr, 0,
+        SceneR<double> sb = s;
+        sb.rgb = rgb + b * rgb_cam_stride;
+        const RayOut<double> r = walk_ray(sb, fl, c.eye, d, c.start, thr, 0.5, max_steps, (int32_t *)nullptr, 0,
                                           rw ? &tm : nullptr);
         const double e0 = r.rgb.x - target[3 * ray], e1 = r.rgb.y - target[3 * ray + 1],
                      e2 = r.rgb.z - target[3 * ray + 2];
@@ -863,9 +869,10 @@ static double loss_f64(const double *points, const double *radii, const double *
 double orc_walk_loss_f64(const double *points, const double *radii, const double *nsigmas, const double *sites,
                          const double *rgb, const double *height, int32_t K, int64_t N, const int32_t *adj,
                          const int32_t *off, int64_t E, const CamIn *cams, int32_t B, double thr, int32_t max_steps,
-                         const int64_t *rays, int64_t n_rays, const float *target, double w_alpha, int32_t nthreads) {
+                         const int64_t *rays, int64_t n_rays, const float *target, double w_alpha, int32_t nthreads,
+                         int64_t rgb_cam_stride) {
     return loss_f64(points, radii, nsigmas, sites, rgb, height, K, N, adj, off, E, cams, B, thr, max_steps, rays,
-                    n_rays, target, w_alpha, nullptr, nthreads);
+                    n_rays, target, w_alpha, nullptr, nthreads, rgb_cam_stride);
 }
 
 // The same loss plus the raster side outputs weighted by rw (one camera):

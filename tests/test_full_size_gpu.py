@@ -203,18 +203,11 @@ def test_262k_cells_16_cameras_finite_differences():
     an = dict(points=g["spheres"][:, :3], radii=g["spheres"][:, 3], nsigmas=g["nsigmas"],
               texel_sites=g["texel_sites"], texel_rgb=g["texel_rgb"], texel_height=g["texel_height"])
     adj_np, off_np = adj.cpu().numpy(), off.cpu().numpy()
-    per = W * H
-    cam_rays = [rays[(rays >= b * per) & (rays < (b + 1) * per)] - b * per for b in range(B)]
 
     def loss():
-        tot = 0.0
-        for b in range(B):
-            if cam_rays[b].size == 0:
-                continue
-            Pb = dict(P, texel_rgb=rgb64[b])
-            tot += orc.walk_loss_f64(Pb, adj_np, off_np, words[b:b + 1], target[b:b + 1], w_alpha=w_alpha,
-                                     rays=cam_rays[b])
-        return tot
+        # every camera with its own colours, in one pass over the sampled rays
+        return orc.walk_loss_f64(dict(P, texel_rgb=rgb64), adj_np, off_np, words, target, w_alpha=w_alpha,
+                                 rays=rays)
 
     touched = np.flatnonzero(np.abs(g["nsigmas"][:, 3]) > 1e-6)
     names = list(P) + ["texel_rgb"]

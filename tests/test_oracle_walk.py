@@ -249,6 +249,40 @@ def test_walk_backward_matches_fp64_central_differences(scene_textured):
     assert rel < 1e-3, (rel, list(zip(picks, a, fd)))
 
 
+def test_fp64_loss_with_per_camera_colours_is_the_sum_over_cameras(scene_textured):
+    """One FP64 loss call over several cameras, each with its own texel
+    colours (texel_rgb (B, N, K, 3)), on sampled rays of all of them equals
+    the sum of one call per camera with that camera's colours; with
+    identical colours it equals the shared-colour call."""
+    sc = scene_textured
+    rng = np.random.default_rng(3)
+    W = H = 24
+    cams = workloads.sphere_cameras(rng, 3, W, H)
+    words = np.concatenate([_cam_words(sc, c) for c in cams])
+    B, N, K = 3, len(sc["radii"]), 8
+    P = dict(points=sc["points"].astype(np.float64), radii=sc["radii"].astype(np.float64),
+             nsigmas=np.concatenate([sc["normals"], sc["density"][:, None]], 1).astype(np.float64),
+             texel_sites=sc["texel_sites"].astype(np.float64), texel_height=sc["texel_height"].astype(np.float64))
+    rgb = rng.uniform(0, 1, (B, N, K, 3))
+    target = rng.uniform(0, 1, (B, H, W, 3)).astype(np.float32)
+    rays = np.sort(rng.choice(B * W * H, 300, replace=False))
+    both = orc.walk_loss_f64(dict(P, texel_rgb=rgb), sc["adjacency"], sc["offsets"], words, target, w_alpha=0.3,
+                             rays=rays)
+    per = 0.0
+    for b in range(B):
+        rb = rays[(rays >= b * W * H) & (rays < (b + 1) * W * H)] - b * W * H
+        per += orc.walk_loss_f64(dict(P, texel_rgb=rgb[b]), sc["adjacency"], sc["offsets"], words[b:b + 1],
+                                 target[b:b + 1], w_alpha=0.3, rays=rb)
+    assert abs(both - per) <= 1e-12 * abs(per)
+    same = orc.walk_loss_f64(dict(P, texel_rgb=np.stack([rgb[0]] * B)), sc["adjacency"], sc["offsets"], words,
+                             target, w_alpha=0.3, rays=rays)
+    shared = orc.walk_loss_f64(dict(P, texel_rgb=rgb[0]), sc["adjacency"], sc["offsets"], words, target,
+                               w_alpha=0.3, rays=rays)
+    assert same == shared
+    with pytest.raises(ValueError):
+        orc.walk_loss_f64(dict(P, texel_rgb=rgb[:2]), sc["adjacency"], sc["offsets"], words, target, rays=rays)
+
+
 # ------------------------------------------------------ walks that loop --
 def _loop_scene(hit_b):
     """Three cells whose exit faces send the rays of a narrow 3x3 camera

@@ -55,15 +55,15 @@ def _drain(server, reqs):
     torch.cuda.synchronize()
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_alone_equals_mixed(served, use_graph):
+@pytest.mark.parametrize("use_graph,backward", [(False, "grouped"), (True, "grouped"), (True, "recorded")])
+def test_alone_equals_mixed(served, use_graph, backward):
     alone = _requests(3, 6)
     for r in alone:
-        srv = RenderServer(DEV, tile_capacity=64, quantum=16, use_graph=use_graph)
+        srv = RenderServer(DEV, tile_capacity=64, quantum=16, use_graph=use_graph, backward=backward)
         srv.add_scene("s", served)
         _drain(srv, [r])
     mixed = _requests(3, 6)
-    srv = RenderServer(DEV, tile_capacity=64, quantum=16, use_graph=use_graph)
+    srv = RenderServer(DEV, tile_capacity=64, quantum=16, use_graph=use_graph, backward=backward)
     srv.add_scene("s", served)
     _drain(srv, mixed)
     assert int(srv.status.item()) == 0
@@ -91,16 +91,19 @@ def test_served_equals_plain_path(served):
             assert np.linalg.norm(x - y) <= 1e-4 * np.linalg.norm(y)
 
 
-@pytest.mark.parametrize("record_cap", [0, 8])
-def test_served_backward_with_and_without_record(served, record_cap):
-    """Forward+backward tiles differentiated by replay (record_cap 0) or from
-    a small record (8: rays with more contributing segments are replayed in
-    the same wave) give the plain path's gradients."""
+@pytest.mark.parametrize("record_cap,backward", [(0, "recorded"), (8, "recorded"), (8, "grouped"), (128, "grouped")])
+def test_served_backward_with_and_without_record(served, record_cap, backward):
+    """Forward+backward tiles differentiated by replay (record_cap 0), from a
+    small record read tile by tile (8: rays with more contributing segments
+    are replayed in the same wave), or grouped by cell over the step (rays
+    past the cap replayed in the same launch) give the plain path's
+    gradients."""
     reqs = [r for r in _requests(11, 9) if r.mode == "forward_backward"]
-    srv = RenderServer(DEV, tile_capacity=256, quantum=64, record_cap=record_cap)
+    srv = RenderServer(DEV, tile_capacity=256, quantum=64, record_cap=record_cap, backward=backward)
     srv.add_scene("s", served)
     _drain(srv, reqs)
     assert int(srv.status.item()) == 0
+    assert srv.backward == (backward if record_cap else "recorded")
     plain = PlainRenderer({"s": served})
     for r in [r for r in _requests(11, 9) if r.mode == "forward_backward"]:
         plain.render(r)
@@ -109,7 +112,7 @@ def test_served_backward_with_and_without_record(served, record_cap):
         for k in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
             x, y = q.grads[k].cpu().numpy(), r.grads[k].cpu().numpy()
             assert np.linalg.norm(x - y) <= 1e-4 * max(np.linalg.norm(y), 1e-30), k
-    if record_cap:
+    if record_cap == 8:
         # the record held some rays and not others: their counts, from a
         # recording forward of the same requests
         from powerfoam import _lib, shading
@@ -272,3 +275,29 @@ def test_multi_scene_admission_mix_equals_plain():
             for k in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
                 x, y = a.grads[k].cpu().numpy(), b.grads[k].cpu().numpy()
                 assert np.linalg.norm(x - y) <= 1e-4 * max(np.linalg.norm(y), 1e-30), (a.rid, k)
+
+
+def test_grouped_step_limits_and_entry_budget(served):
+    """The grouped serving step with one forward+backward request's cells per
+    step (max_buckets = N), eight forward+backward tiles per step and a sort
+    budget of two entries per slot (most rays with segments go to the
+    replay list) still gives the plain path's colours and gradients, with
+    forward requests in the same steps."""
+    n = served.num_cells
+    reqs = _requests(21, 9)
+    srv = RenderServer(DEV, tile_capacity=64, quantum=16, bwd_capacity=8, max_buckets=n, entries_per_slot=2)
+    srv.add_scene("s", served)
+    with pytest.raises(ValueError):
+        big = RenderServer(DEV, max_buckets=n - 1)
+        big.add_scene("s", served)
+    _drain(srv, reqs)
+    assert int(srv.status.item()) == 0 and srv.backward == "grouped"
+    plain = PlainRenderer({"s": served})
+    for r in _requests(21, 9):
+        plain.render(r)
+        q = next(x for x in reqs if x.rid == r.rid)
+        assert torch.equal(q.color, r.color) and torch.equal(q.log_t, r.log_t)
+        if r.mode == "forward_backward":
+            for k in ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height"):
+                x, y = q.grads[k].cpu().numpy(), r.grads[k].cpu().numpy()
+                assert np.linalg.norm(x - y) <= 1e-4 * max(np.linalg.norm(y), 1e-30), k

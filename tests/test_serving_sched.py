@@ -15,10 +15,15 @@ def _fake(ntiles_list):
     return reqs
 
 
-def _plan(active, capacity, quantum, oldest_share=0.0, bwd_weight=1):
+def _plan(active, capacity, quantum, oldest_share=0.0, bwd_weight=1, grouped=None):
     srv = RenderServer.__new__(RenderServer)
     srv.active, srv.capacity, srv.quantum = active, capacity, quantum
     srv.oldest_share, srv.bwd_weight = oldest_share, bwd_weight
+    srv.backward, srv._records = "recorded", None
+    if grouped is not None:  # (bwd_capacity, max_buckets, {scene: cells})
+        srv.backward, srv._records = "grouped", []
+        srv.bwd_capacity, srv.max_buckets = grouped[0], grouped[1]
+        srv.scenes = {k: types.SimpleNamespace(num_cells=n) for k, n in grouped[2].items()}
     return srv._schedule()
 
 
@@ -64,3 +69,32 @@ def test_oldest_share_and_backward_weight():
     # 256 (quantum / 4) for the backward request, 1024 for the forward one,
     # then the remaining 2048 of capacity buys the oldest 512 more tiles
     assert takes == {0: 768, 1: 1024}
+
+
+def test_grouped_backward_limits():
+    # forward+backward tiles per step stay within bwd_capacity, and a step's
+    # forward+backward requests' cells within max_buckets; forward requests
+    # take the rest of the capacity
+    a = _fake([5000, 5000, 5000, 3000])
+    for r, sc in zip(a[:3], ("x", "y", "x")):
+        r.mode, r.scene = "forward_backward", sc
+    cells = {"x": 600, "y": 500}
+    takes = {p[0].rid: p[2] for p in _plan(a, 4096, 1024, grouped=(1536, 1100, cells))}
+    # request 0 (600 cells) and 1 (500) fit the 1100 buckets, request 2 waits
+    assert 2 not in takes
+    assert takes[0] + takes[1] <= 1536 and takes[0] == 1024 and takes[1] == 512
+    assert takes[3] == 4096 - 1536
+    # the same requests across steps: every tile exactly once, limits held
+    seen = {r.rid: [] for r in a}
+    active = list(a)
+    while active:
+        plan = _plan(active, 4096, 1024, grouped=(1536, 1100, cells))
+        assert sum(p[2] for p in plan if getattr(p[0], "mode", "") == "forward_backward") <= 1536
+        assert sum(cells[p[0].scene] for p in plan if getattr(p[0], "mode", "") == "forward_backward") <= 1100
+        for r, b, take in plan:
+            assert b == r._next_tile
+            seen[r.rid].extend(range(b, b + take))
+            r._next_tile = b + take
+        active = [r for r in active if r._next_tile < r.ntiles]
+    for r in a:
+        assert seen[r.rid] == list(range(r.ntiles))
