@@ -60,9 +60,12 @@ def main():
                     help="power-diagram adjacency from the GPU builder or the CPU Qhull lift (same edges)")
     ap.add_argument("--serve-quantum", type=int, default=1024, help="tiles per request per step before leftovers")
     ap.add_argument("--serve-streams", type=int, default=3, help="server streams (table buffers)")
+    ap.add_argument("--serve-backward", choices=("grouped", "recorded"), default="grouped",
+                    help="server's forward+backward tiles: grouped by (request, cell) over the step, or tile by tile "
+                         "from the tile's record")
     ap.add_argument("--serve-record-cap", type=int, default=128,
                     help="segments per ray the server records for forward+backward tiles (0 = replay)")
-    ap.add_argument("--serve-oldest-share", type=float, default=0.0,
+    ap.add_argument("--serve-oldest-share", type=float, default=1.0,
                     help="share of a step's capacity the oldest active request may take")
     ap.add_argument("--record-cap", type=int, default=128,
                     help="contributing segments per ray the forward records for the backward (0 = replay)")
@@ -71,8 +74,13 @@ def main():
                          "or checkpoint-and-replay without a record")
     ap.add_argument("--c3", type=int, default=1,
                     help="also time the 16-camera differentiation configuration (262,144 cells, 800x800, 0 = skip)")
+    ap.add_argument("--c1", type=int, default=1,
+                    help="also run the plain forward configuration (4,096 cells, 128x128) in full on the GPU and "
+                         "the CPU (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (reference kernel for HBM byte counters)")
+    ap.add_argument("--sweep-traces", type=int, default=3, help="--serve-sweep: independent traces per load")
+    ap.add_argument("--sweep-requests", type=int, default=480, help="--serve-sweep: requests per trace")
     ap.add_argument("--serve-sweep", action="store_true",
                     help="only run both serving traces at loads 0.3/0.6/0.9 and at saturation, server and plain "
                          "path, and print that as one JSON object")
@@ -218,7 +226,7 @@ def main():
     # algorithmic bytes from the kernels' own per-ray counters (untimed pass)
     shading.sh_forward(coeffs, 3, sites, cams_dev, out=texel_rgb)
     f = rt.launch_walk_forward(scene, cams_dev, W, H, want_depth=False, want_sequences=True, want_bytes=True,
-                               record_cap=args.record_cap)
+                               record_cap=args.record_cap, want_ray_status=True)
     gc = 2.0 * (f["color"] - target)
     for k in GRAD_KEYS:
         grads[k].zero_()
@@ -237,6 +245,10 @@ def main():
     bwd_bytes = float(b["alg_bytes"].sum().item())
     visits = f["visits"].double()
     status = int(f["status"].item()) | int(b["status"].item())
+    # each status bit with the number of rays that set it in the forward
+    rs = f["ray_status"]
+    status_rays = {name: int(((rs & bit) != 0).sum().item())
+                   for name, bit in (("step_cap", 1), ("cycle", 4), ("bad_record", 8)) if status & bit}
     fwd_ms = float(np.median(times["fwd"]))
     bwd_ms = float(np.median(times["bwd"]))
     dom = "walk_bwd" if bwd_ms >= fwd_ms else "walk_fwd"
@@ -244,7 +256,8 @@ def main():
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     log(f"step {dt * 1e3:.2f} ms = {mrays:.1f} Mrays/s; walk fwd {fwd_ms:.2f} ms "
         f"({fwd_bytes / fwd_ms / 1e6:.0f} GB/s alg), bwd {bwd_ms:.2f} ms ({bwd_bytes / bwd_ms / 1e6:.0f} GB/s alg); "
-        f"visits/ray mean {visits.mean().item():.1f} max {int(visits.max().item())}; status {status}")
+        f"visits/ray mean {visits.mean().item():.1f} max {int(visits.max().item())}; status {status} "
+        f"(rays: {status_rays})")
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -255,6 +268,7 @@ def main():
         raster = raster_bench(sc, pts, rad, texel_rgb[0], target, cam, args)
 
     c3 = c3_bench(args) if args.c3 else None
+    c1 = c1_bench(args) if args.c1 else None
 
     serving = serving_multi = None
     if args.serve_requests > 0:
@@ -297,6 +311,7 @@ def main():
             "walk_bwd_alg_bytes": bwd_bytes,
             "record_segments": segs,
             "status": status,
+            "status_rays": status_rays,
             "backward_forms": forms,
         },
         "roofline": {
@@ -311,6 +326,7 @@ def main():
         "cpu_baseline": cpu,
         "raster": raster,
         "differentiation_16_cameras": c3,
+        "plain_forward_4096_cells": c1,
         "serving": serving,
         "serving_multi": serving_multi,
     }
@@ -542,6 +558,9 @@ def plain_service(scenes, make):
     return warm, (time.perf_counter() - t0) / len(probe)
 
 
+_SERVERS = {}
+
+
 def serve_trace(scenes, make, args, label, load=None, calibration=None):
     """Serve one trace with the continuous-batching server and with the
     plain per-request path (one request at a time, FIFO).  Poisson arrivals
@@ -581,13 +600,24 @@ def serve_trace(scenes, make, args, label, load=None, calibration=None):
     p_trace = copy.deepcopy(trace)
     p_wall = plain.run(p_trace)
     p_stats = latency_stats(p_trace)
-    srv = RenderServer("cuda", memory_budget=64 << 30, tile_capacity=args.serve_capacity, quantum=args.serve_quantum,
-                       oldest_share=args.serve_oldest_share, streams=args.serve_streams,
-                       record_cap=args.serve_record_cap, max_wait=1e9 if load == "saturation" else 2.0)
-    for name, scene in scenes.items():
-        srv.add_scene(name, scene)
-    srv.on_finish = consume
-    srv.run(copy.deepcopy(warm))  # warm-up (graph capture, the allocator's cache)
+    # one server per scene set and configuration, warmed once and kept across
+    # loads and traces, as a serving process keeps its server (the plain
+    # path's allocator cache is warm the same way)
+    key = (id(scenes), args.serve_capacity, args.serve_quantum, args.serve_oldest_share, args.serve_streams,
+           args.serve_record_cap, args.serve_backward)
+    srv = _SERVERS.get(key)
+    if srv is None:
+        srv = RenderServer("cuda", memory_budget=64 << 30, tile_capacity=args.serve_capacity,
+                           quantum=args.serve_quantum, oldest_share=args.serve_oldest_share,
+                           streams=args.serve_streams, record_cap=args.serve_record_cap, max_wait=1e9,
+                           backward=args.serve_backward)
+        for name, scene in scenes.items():
+            srv.add_scene(name, scene)
+        srv.on_finish = consume
+        srv.run(copy.deepcopy(warm))  # warm-up (graph capture, the allocator's cache)
+        _SERVERS.clear()
+        _SERVERS[key] = srv
+    srv.max_wait = 1e9 if load == "saturation" else 2.0
     s_trace = copy.deepcopy(trace)
     srv.finished, srv.rejected = [], []
     s_wall = srv.run(s_trace)
@@ -687,6 +717,7 @@ def serve_bench(args):
     """Serving form, one resident scene (single_scene_trace) at --serve-load."""
     scenes, make, workload = single_scene_trace(args)
     out = serve_trace(scenes, make, args, "serving")
+    _SERVERS.clear()
     out["workload"] = workload
     return out
 
@@ -696,6 +727,7 @@ def serve_multi_bench(args):
     --serve-load."""
     scenes, make, workload, sizes = multi_scene_trace(args)
     out = serve_trace(scenes, make, args, "serving (multi-scene)")
+    _SERVERS.clear()
     out["scene_cells"] = sizes
     out["workload"] = workload
     return out
@@ -707,21 +739,39 @@ SWEEP_LOADS = (0.3, 0.6, 0.9, "saturation")
 def serve_sweep(args):
     """Both serving traces at offered loads 0.3, 0.6 and 0.9 of the plain
     path's capacity and at saturation (every request submitted at once):
-    p50/p99 latency and throughput of the server and the plain path.
-    Prints one JSON object (not the benchmark line)."""
+    p50/p99 latency and throughput of the server and the plain path, on
+    --sweep-traces independent traces of --sweep-requests requests per load
+    (a trace's p99 swings with its bursts), each row and the mean over the
+    traces.  Prints one JSON object (not the benchmark line)."""
+    import numpy as np
     import torch
 
-    out = {"loads": [str(x) for x in SWEEP_LOADS]}
+    out = {"loads": [str(x) for x in SWEEP_LOADS], "traces_per_load": args.sweep_traces,
+           "requests_per_trace": args.sweep_requests, "serve_backward": args.serve_backward}
+    args.serve_requests = args.sweep_requests
+    keys = ("p50_ms", "p99_ms", "mrays_per_s", "plain_p50_ms", "plain_p99_ms", "plain_mrays_per_s")
     for name, builder in (("single_scene", single_scene_trace), ("multi_scene", multi_scene_trace)):
         if name == "multi_scene" and args.serve_scenes <= 0:
             continue
         built = builder(args)
         scenes, make, workload = built[:3]
         cal = plain_service(scenes, make)
-        rows = [serve_trace(scenes, make, args, f"{name} load {ld}", load=ld, calibration=cal) for ld in SWEEP_LOADS]
-        out[name] = {"workload": workload, "rows": rows}
+        rows, means = [], []
+        for ld in SWEEP_LOADS:
+            runs = [serve_trace(scenes, make, args, f"{name} load {ld} trace {k}", load=ld, calibration=cal)
+                    for k in range(args.sweep_traces)]
+            rows.extend(runs)
+            mean = {"load": ld, **{k: round(float(np.mean([r[k] for r in runs])), 3) for k in keys}}
+            fb = [r["forward_backward"] for r in runs if "forward_backward" in r]
+            if fb:
+                mean["forward_backward"] = {k: round(float(np.mean([f[k] for f in fb])), 3)
+                                            for k in ("p50_ms", "p99_ms", "plain_p50_ms", "plain_p99_ms")}
+            means.append(mean)
+            log(f"{name} load {ld} mean over {len(runs)} traces: {mean}")
+        out[name] = {"workload": workload, "rows": rows, "means": means}
         if len(built) > 3:
             out[name]["scene_cells"] = built[3]
+        _SERVERS.clear()
         del scenes, make, built, cal
         torch.cuda.empty_cache()
     print(json.dumps(out), flush=True)
@@ -756,6 +806,69 @@ def copy_calibration(nbytes=1 << 30, reps=5):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes": nbytes, "launches": reps}), flush=True)
+
+
+def c1_bench(args):
+    """The plain forward configuration in full: 4,096 cells uniform in
+    [-1, 1]^3, weights U[0, 0.01], density lognormal(0, 1), one colour per
+    cell, the Qhull-lift adjacency, one 128x128 60-degree camera at distance
+    3 looking at the origin, seed 0.  Every ray on the GPU (median of 200
+    launches) and on the host's cores with the oracle; sequences compared
+    bit for bit and colours to 1e-4."""
+    import numpy as np
+    import torch
+
+    from oracle import oracle as orc
+    from powerfoam import raytrace as rt
+    from powerfoam import workloads
+    from powerfoam.adjacency import build_power_adjacency
+    from powerfoam.camera import pack_cameras
+
+    sc = workloads.uniform_scene(4096, seed=0)
+    adj, off = build_power_adjacency(torch.from_numpy(sc["points"]), torch.from_numpy(sc["radii"]))
+    adj, off = adj.numpy(), off.numpy()
+    cam = workloads.front_camera(128, 128)
+    words = pack_cameras([cam], [orc.start_cell(sc["points"], sc["radii"], cam.eye.numpy())])
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    pts, rad, adj_d, off_d = t(sc["points"]), t(sc["radii"]), t(adj), t(off)
+    faces = rt.walk_faces(pts, rad, adj_d, off_d)
+    scene = rt.WalkScene(pts, rad, t(sc["density"]), t(sc["normals"]), t(sc["texel_sites"]), t(sc["texel_rgb"]),
+                         t(sc["texel_height"]), adj_d, off_d, faces)
+    cams = t(words)
+    for _ in range(5):
+        rt.launch_walk_forward(scene, cams, 128, 128, want_depth=False)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ms = []
+    for _ in range(200):
+        ev[0].record()
+        rt.launch_walk_forward(scene, cams, 128, 128, want_depth=False)
+        ev[1].record()
+        torch.cuda.synchronize()
+        ms.append(ev[0].elapsed_time(ev[1]))
+    g = rt.launch_walk_forward(scene, cams, 128, 128, want_sequences=True)
+    torch.cuda.synchronize()
+    fh = faces.cpu().view(torch.int16).numpy().view(np.uint16)
+    cs = orc.Scene(sc["points"], sc["radii"], sc["density"], sc["normals"], sc["texel_sites"], sc["texel_rgb"],
+                   sc["texel_height"], adj, off, fh)
+    nth = orc.num_threads()
+    orc.walk_forward(cs, words, nthreads=nth)
+    cpu_s = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        c = orc.walk_forward(cs, words, nthreads=nth)
+        cpu_s.append(time.perf_counter() - t0)
+    same = bool(np.array_equal(g["visits"].cpu().numpy(), c["visits"]) and
+                np.array_equal(g["seq_hash"].cpu().numpy().view(np.uint64), c["seq_hash"]))
+    err = float(np.abs(g["color"].cpu().numpy() - c["color"]).max())
+    gms, cs_ = float(np.median(ms)), float(np.median(cpu_s))
+    out = {"workload": "plain walk forward, 4096 uniform cells, 128x128, 60 deg, seed 0 (every ray)",
+           "rays": 128 * 128, "gpu_ms": round(gms, 4), "gpu_mrays_per_s": round(128 * 128 / (gms * 1e-3) / 1e6, 2),
+           "cpu_ms": round(cs_ * 1e3, 3), "cpu_mrays_per_s": round(128 * 128 / cs_ / 1e6, 4), "cpu_cores": nth,
+           "cpu_kind": "port", "sequences_equal": same, "max_abs_rgb": err,
+           "mean_visits_per_ray": round(float(c["visits"].mean()), 2)}
+    log(f"plain forward 4096 cells 128x128: GPU {gms:.3f} ms, CPU {cs_ * 1e3:.2f} ms on {nth} threads; "
+        f"sequences equal {same}, max |rgb| {err:.1e}")
+    return out
 
 
 def cpu_baseline(sc, adj, off, faces, texel_rgb, cams_dev, fwd, gc, seconds):

@@ -5,7 +5,7 @@
 // The ABI's struct sizes, which bindings mirror field for field.
 static_assert(sizeof(pf_camera) == 48, "pf_camera layout");
 static_assert(sizeof(pf_scene) == 104, "pf_scene layout");
-static_assert(sizeof(pf_walk_out) == 80, "pf_walk_out layout");
+static_assert(sizeof(pf_walk_out) == 88, "pf_walk_out layout");
 static_assert(sizeof(pf_walk_record) == 40, "pf_walk_record layout");
 static_assert(sizeof(pf_grads) == 48, "pf_grads layout");
 static_assert(sizeof(pf_serve_scene) == 120, "pf_serve_scene layout");

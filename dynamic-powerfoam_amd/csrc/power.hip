@@ -1,7 +1,8 @@
 // power.hip — power-diagram adjacency on the GPU (the graph that
 // build_power_adjacency, benchmark.py:25-83, takes from a Qhull lift).
 //
-// One thread per cell i clips a box by the power bisectors
+// One wavefront per cell i (power_cells_wave_kernel) clips a box by the
+// power bisectors
 //   y . d_j <= 0.5 (L_j - L_i) - p_i . d_j,   d_j = p_j - p_i,  L = |p|^2 - r^2,
 // in FP64 coordinates y centred at p_i.  The cell is a convex polytope kept
 // as its dual triangulation: each triangle is a vertex (three planes), with
@@ -339,125 +340,12 @@ __device__ int write_row(const Poly &P, unsigned long long keep, int max_nb, int
     return flag;
 }
 
-// One cell per thread.  lists (N, max_nb) int32 sorted ascending, counts (N),
-// flags (N): bit 0 the cell touches the box boundary, bit 1 capacity
-// overflow (the row is incomplete), bit 2 the cell is empty (hidden site).
-// lifted (N): the lifted coordinate L = |p|^2 - r^2 as the caller computed
-// it (benchmark.py:29-32 computes it in f32).  slack >= |L - (|p|^2 - r^2)|
-// bounds that rounding.
-__global__ __launch_bounds__(64) void power_cells_kernel(const float *pts, const float *lifted, int64_t N,
-                                                         PowerGrid g, const int64_t *gstart, const int *items,
-                                                         const unsigned char *occ, float3 blo, float3 bhi,
-                                                         float rmax, float slack, int max_nb, int *lists,
-                                                         int *counts, int *flags) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    Poly P;
-    const d3 pi = dd(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
-    const double Li = (double)lifted[i];
-    init_box(P, dd(blo.x - pi.x, blo.y - pi.y, blo.z - pi.z), dd(bhi.x - pi.x, bhi.y - pi.y, bhi.z - pi.z));
-    // the same bucketing as pgrid_count_kernel
-    const int ix = bucket_of(pts[3 * i], g.ox, g.cell, g.nx[0]);
-    const int iy = bucket_of(pts[3 * i + 1], g.oy, g.cell, g.ny[0]);
-    const int iz = bucket_of(pts[3 * i + 2], g.oz, g.cell, g.nz[0]);
-    const int kmax = max(g.nx[0], max(g.ny[0], g.nz[0]));
-    auto offset = [&](int j) {
-        return dd((double)pts[3 * j] - pi.x, (double)pts[3 * j + 1] - pi.y, (double)pts[3 * j + 2] - pi.z);
-    };
-    auto rhs_of = [&](int j, d3 dj) { return 0.5 * ((double)lifted[j] - Li) - ddot(pi, dj); };
-    // 1. the kNear nearest sites in the order (|d_j|^2, j), by grid rings:
-    // after ring k every site closer than (k - 0.01) buckets has been seen
-    // (a hundredth of a bucket is left for the float bucketing)
-    double kd[kNear];
-    int kj[kNear];
-    int kc = 0;
-    for (int k = 0; k <= kmax; ++k) {
-        for_shell(g, gstart, items, ix, iy, iz, k, [&](int j) {
-            if (j == i) return true;
-            const d3 dj = offset(j);
-            const double d2 = ddot(dj, dj);
-            if (d2 <= 1e-24) return true;  // coincident sites: no bisector
-            if (kc == kNear && !(d2 < kd[kNear - 1] || (d2 == kd[kNear - 1] && j < kj[kNear - 1]))) return true;
-            int w = kc < kNear ? kc++ : kNear - 1;
-            for (; w > 0 && (kd[w - 1] > d2 || (kd[w - 1] == d2 && kj[w - 1] > j)); --w) {
-                kd[w] = kd[w - 1];
-                kj[w] = kj[w - 1];
-            }
-            kd[w] = d2;
-            kj[w] = j;
-            return true;
-        });
-        const double D = ((double)k - 0.01) * (double)g.cell;
-        if (kc == kNear && k >= 1 && D * D > kd[kNear - 1]) break;
-    }
-    // 2. clip by them nearest first; later planes mostly miss the cell's ball
-    for (int t = 0; t < kc && P.nt > 0 && !P.overflow; ++t) {
-        const d3 dj = offset(kj[t]);
-        const double rhs = rhs_of(kj[t], dj);
-        if (!misses_ball(dj, rhs, P.r2)) clip(P, dj, rhs, kj[t]);
-    }
-    // 3. certify the vertices.  With fewer than kNear other sites every site
-    // has been clipped; otherwise the sites not clipped lie at least
-    // sqrt(kd[kNear - 1]) from p_i, and a site that cuts vertex v lies within
-    // rho of v, rho^2 = |v|^2 + r_max^2 - r_i^2 (+ the lift's rounding).
-    const double rm2 = (double)rmax * (double)rmax + 2.0 * (double)slack;
-    const double ri2 = ddot(pi, pi) - Li;
-    const double dK = kc == kNear ? sqrt(kd[kNear - 1]) : 0.0;
-    for (int round = 0; kc == kNear && P.nt > 0 && !P.overflow; ++round) {
-        int t = 0;
-        while (t < P.nt && P.cert[t]) ++t;
-        if (t == P.nt) break;
-        if (round == kMaxRounds) {
-            P.overflow = true;
-            break;
-        }
-        const d3 v = P.v[t];
-        const double v2 = ddot(v, v);
-        const double rho2 = (v2 + rm2 - ri2) * (1.0 + 1e-9) + 1e-18;
-        if (rho2 <= 0.0) {  // no site reaches v
-            P.cert[t] = 1;
-            continue;
-        }
-        const double rho = sqrt(rho2);
-        if ((sqrt(v2) + rho) * (1.0 + 1e-9) < dK) {  // the ball holds clipped sites only
-            P.cert[t] = 1;
-            continue;
-        }
-        bool cut = false;
-        ball_query(g, occ, gstart, items, dd(pi.x + v.x, pi.y + v.y, pi.z + v.z), rho, [&](int j) {
-            if (j == i) return true;
-            const d3 dj = offset(j);
-            const double d2 = ddot(dj, dj);
-            if (d2 <= 1e-24) return true;
-            if (d2 < kd[kNear - 1] || (d2 == kd[kNear - 1] && j <= kj[kNear - 1])) return true;  // clipped in 2.
-            const double rhs = rhs_of(j, dj);
-            if (!(ddot(dj, v) > rhs)) return true;  // the same test as clip()'s
-            for (int p = 0; p < P.np; ++p)
-                if (P.nb[p] == j) return true;  // applied already: rounding only
-            clip(P, dj, rhs, j);
-            cut = true;
-            return false;
-        });
-        if (!cut) P.cert[t] = 1;
-    }
-    int flag = P.overflow ? 2 : 0;
-    if (P.nt == 0) flag |= 4;
-    // neighbours: planes referenced by surviving triangles
-    unsigned long long used = 0;
-    for (int t = 0; t < P.nt; ++t)
-        for (int m = 0; m < 3; ++m) used |= 1ull << P.tr[t][m];
-    int c = 0;
-    flag |= write_row(P, used, max_nb, lists + i * (int64_t)max_nb, c);
-    counts[i] = c;
-    flags[i] = flag;
-}
-
 // Faces outside the box.  Every point outside box B is owned by a site whose
 // cell meets B's boundary, so the diagram outside B is that of those sites
 // (hull) alone.  One thread per hull site h clips a far box by the
 // bisectors of all other hull sites and keeps the planes whose face has a
 // vertex outside B (a convex face with every vertex inside B lies inside
-// B).  Output as power_cells_kernel (lists (nh, max_nb) of global ids).
+// B).  Output as power_cells_wave_kernel (lists (nh, max_nb) of global ids).
 __global__ __launch_bounds__(64) void power_hull_kernel(const float *pts, const float *lifted, const int *hull,
                                                         int nh, float3 blo, float3 bhi, float far, int max_nb,
                                                         int *lists, int *counts, int *flags) {
@@ -495,6 +383,576 @@ __global__ __launch_bounds__(64) void power_hull_kernel(const float *pts, const 
     const int flag = (P.overflow ? 2 : 0) | (write_row(P, outside, max_nb, lists + (int64_t)h * max_nb, c) & 2);
     counts[h] = c;
     flags[h] = flag;
+}
+
+// ------------------------------------------------------ one wave per cell --
+// power_cells_wave_kernel: one 64-lane wavefront per cell, the polytope in
+// the wave's LDS (9 KB).  (One thread per cell kept a 7 KB polytope per
+// thread in scratch memory: 300 GB of traffic and 355 ms at 1M cells.)
+//   * A clip tests up to three triangles per lane, marks the dropped
+//     triangles' directed edges in a 64 x 64 bit matrix (a boundary edge is
+//     a dropped edge whose twin is not), places the new fan triangles by a
+//     prefix sum over the lanes in the serial clip's order and compacts the
+//     kept triangles in order: the polytope after each clip is the serial
+//     clip's, triangle for triangle.
+//   * The kNear nearest sites: each grid shell's sites, 64 at a time, sorted
+//     across the lanes (bitonic, by (|d|^2, index)) and merged into the
+//     sorted best list by rank.
+//   * Certification: the cheap ball tests for every uncertified vertex at
+//     once; the ball queries of the rest one vertex per lane, and the sites
+//     they find clipped one after the other (a clip that no longer cuts is
+//     skipped), until every vertex is certified.
+struct WavePoly {
+    double pl[kMaxPlanes][4];           // n.x n.y n.z d
+    int nb[kMaxPlanes];
+    unsigned int tr[kMaxTris];           // a | b << 8 | c << 16 | certified << 24
+    double v[kMaxTris][3];
+    unsigned long long edge[kMaxPlanes];  // dropped directed edges: bit b of row a
+    double kd[kNear];                    // nearest sites, ascending (|d|^2, j)
+    int kj[kNear];
+    double cd[64];                       // a sorted chunk of candidates
+    int cj[64];
+    int cnt[256];                        // shell segment: sites per position (prefix)
+};
+
+struct WaveState {
+    int np, nt;
+    double r2;
+    bool overflow;
+};
+
+__device__ __forceinline__ int tri_a(unsigned t) { return (int)(t & 0xff); }
+__device__ __forceinline__ int tri_b(unsigned t) { return (int)((t >> 8) & 0xff); }
+__device__ __forceinline__ int tri_c(unsigned t) { return (int)((t >> 16) & 0xff); }
+__device__ __forceinline__ unsigned tri_pack(int a, int b, int c, int cert) {
+    return (unsigned)a | ((unsigned)b << 8) | ((unsigned)c << 16) | ((unsigned)cert << 24);
+}
+__device__ __forceinline__ d3 wv(const WavePoly &P, int t) { return dd(P.v[t][0], P.v[t][1], P.v[t][2]); }
+
+__device__ __forceinline__ double wave_max_d(double x) {
+    for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+    return x;
+}
+__device__ __forceinline__ int wave_excl_scan(int v, int &total) {
+    const int lane = __lane_id();
+    int incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+    }
+    total = __shfl(incl, 63, 64);
+    return incl - v;
+}
+
+__device__ __forceinline__ d3 wvertex_of(const WavePoly &P, int a, int b, int c) {
+    const d3 An = dd(P.pl[a][0], P.pl[a][1], P.pl[a][2]), Bn = dd(P.pl[b][0], P.pl[b][1], P.pl[b][2]),
+             Cn = dd(P.pl[c][0], P.pl[c][1], P.pl[c][2]);
+    const double Ad = P.pl[a][3], Bd = P.pl[b][3], Cd = P.pl[c][3];
+    const d3 bc = dcross(Bn, Cn), ca = dcross(Cn, An), ab = dcross(An, Bn);
+    const double det = ddot(An, bc);
+    const double s = 1.0 / det;
+    return dd((Ad * bc.x + Bd * ca.x + Cd * ab.x) * s, (Ad * bc.y + Bd * ca.y + Cd * ab.y) * s,
+              (Ad * bc.z + Bd * ca.z + Cd * ab.z) * s);
+}
+
+__device__ __forceinline__ void wradius(const WavePoly &P, WaveState &S) {
+    double m = 0.0;
+    for (int t = __lane_id(); t < S.nt; t += 64) m = fmax(m, ddot(wv(P, t), wv(P, t)));
+    S.r2 = wave_max_d(m);
+}
+
+// init_box, by the lanes: the same planes and triangles in the same order.
+__device__ __forceinline__ void winit_box(WavePoly &P, WaveState &S, d3 lo, d3 hi) {
+    const int lane = __lane_id();
+    if (lane < 6) {
+        const int ax = lane >> 1;
+        const bool neg = lane & 1;
+        const double hv = ax == 0 ? hi.x : (ax == 1 ? hi.y : hi.z), lv = ax == 0 ? lo.x : (ax == 1 ? lo.y : lo.z);
+        P.pl[lane][0] = ax == 0 ? (neg ? -1.0 : 1.0) : 0.0;
+        P.pl[lane][1] = ax == 1 ? (neg ? -1.0 : 1.0) : 0.0;
+        P.pl[lane][2] = ax == 2 ? (neg ? -1.0 : 1.0) : 0.0;
+        P.pl[lane][3] = neg ? -lv : hv;
+        P.nb[lane] = kBoxPlane;
+    }
+    if (lane < 8) {
+        const int sx = lane >> 2, sy = (lane >> 1) & 1, sz = lane & 1;
+        const int X = sx ? 1 : 0, Y = sy ? 3 : 2, Z = sz ? 5 : 4;
+        const bool positive = ((sx + sy + sz) & 1) == 0;
+        P.tr[lane] = tri_pack(positive ? X : Y, positive ? Y : X, Z, 0);
+        P.v[lane][0] = sx ? lo.x : hi.x;
+        P.v[lane][1] = sy ? lo.y : hi.y;
+        P.v[lane][2] = sz ? lo.z : hi.z;
+    }
+    if (lane < kMaxPlanes) P.edge[lane] = 0ull;
+    S.np = 6;
+    S.nt = 8;
+    S.overflow = false;
+    __syncthreads();
+    wradius(P, S);
+}
+
+// compact_planes: drops planes no triangle references.
+__device__ void wcompact_planes(WavePoly &P, WaveState &S) {
+    const int lane = __lane_id();
+    unsigned long long used = 0;
+    for (int t = lane; t < S.nt; t += 64) {
+        const unsigned tr = P.tr[t];
+        used |= (1ull << tri_a(tr)) | (1ull << tri_b(tr)) | (1ull << tri_c(tr));
+    }
+    for (int off = 32; off > 0; off >>= 1) used |= __shfl_xor(used, off, 64);
+    double pl[4] = {0, 0, 0, 0};
+    int nb = 0;
+    const bool mine = lane < S.np && ((used >> lane) & 1);
+    if (mine) {
+        for (int q = 0; q < 4; ++q) pl[q] = P.pl[lane][q];
+        nb = P.nb[lane];
+    }
+    __syncthreads();
+    if (mine) {
+        const int to = __builtin_popcountll(used & ((1ull << lane) - 1));
+        for (int q = 0; q < 4; ++q) P.pl[to][q] = pl[q];
+        P.nb[to] = nb;
+    }
+    for (int t = lane; t < S.nt; t += 64) {
+        const unsigned tr = P.tr[t];
+        auto re = [&](int p) { return __builtin_popcountll(used & ((1ull << p) - 1)); };
+        P.tr[t] = tri_pack(re(tri_a(tr)), re(tri_b(tr)), re(tri_c(tr)), (int)(tr >> 24));
+    }
+    S.np = __builtin_popcountll(used);
+    __syncthreads();
+}
+
+// clip(): n . y <= d, plane neighbour nbr.  All lanes call it.
+__device__ bool wclip(WavePoly &P, WaveState &S, d3 n, double d, int nbr) {
+    const int lane = __lane_id();
+    const int nt = S.nt;
+    unsigned long long out[3];
+    int nout = 0;
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+        const int t = 64 * w + lane;
+        const bool o = t < nt && ddot(n, wv(P, t)) > d;
+        out[w] = __ballot(o);
+        nout += __builtin_popcountll(out[w]);
+    }
+    if (nout == 0) return false;
+    if (nout == nt) {  // nothing of the cell survives: a hidden site
+        S.nt = 0;
+        return true;
+    }
+    if (S.np == kMaxPlanes) wcompact_planes(P, S);
+    if (S.np == kMaxPlanes) {
+        S.overflow = true;
+        return false;
+    }
+    const int p = S.np;
+    if (lane == 0) {
+        P.pl[p][0] = n.x;
+        P.pl[p][1] = n.y;
+        P.pl[p][2] = n.z;
+        P.pl[p][3] = d;
+        P.nb[p] = nbr;
+    }
+    S.np = p + 1;
+    // the dropped triangles' directed edges
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+        if ((out[w] >> lane) & 1) {
+            const unsigned tr = P.tr[64 * w + lane];
+            const int a = tri_a(tr), b = tri_b(tr), c = tri_c(tr);
+            atomicOr(&P.edge[a], 1ull << b);
+            atomicOr(&P.edge[b], 1ull << c);
+            atomicOr(&P.edge[c], 1ull << a);
+        }
+    }
+    __syncthreads();
+    // boundary edges (twin not dropped), in the serial order: triangle, then edge
+    int base = nt;
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+        const bool dropped = (out[w] >> lane) & 1;
+        int a[3] = {0, 0, 0}, b[3] = {0, 0, 0}, cnt = 0, bm = 0;
+        if (dropped) {
+            const unsigned tr = P.tr[64 * w + lane];
+            const int x[3] = {tri_a(tr), tri_b(tr), tri_c(tr)};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                a[k] = x[k];
+                b[k] = x[(k + 1) % 3];
+                if (!((P.edge[b[k]] >> a[k]) & 1ull)) {
+                    bm |= 1 << k;
+                    ++cnt;
+                }
+            }
+        }
+        int total = 0;
+        int at = base + wave_excl_scan(cnt, total);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if ((bm >> k) & 1) {
+                if (at < kMaxTris) {
+                    P.tr[at] = tri_pack(a[k], b[k], p, 0);
+                    const d3 vv = wvertex_of(P, a[k], b[k], p);
+                    P.v[at][0] = vv.x;
+                    P.v[at][1] = vv.y;
+                    P.v[at][2] = vv.z;
+                }
+                ++at;
+            }
+        }
+        base += total;
+    }
+    __syncthreads();
+    // clear the edge rows
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+        if ((out[w] >> lane) & 1) {
+            const unsigned tr = P.tr[64 * w + lane];
+            P.edge[tri_a(tr)] = 0ull;
+            P.edge[tri_b(tr)] = 0ull;
+            P.edge[tri_c(tr)] = 0ull;
+        }
+    }
+    if (base > kMaxTris) {
+        __syncthreads();
+        S.overflow = true;
+        return false;
+    }
+    // drop the clipped triangles, keeping the order of the rest
+    unsigned tr_keep[3];
+    double vk[3][3];
+    int to[3];
+    bool keep[3];
+    const int below0 = __builtin_popcountll(out[0]), below1 = below0 + __builtin_popcountll(out[1]);
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+        const int t = 64 * w + lane;
+        keep[w] = t < base && !((out[w] >> lane) & 1);
+        to[w] = 0;
+        if (keep[w]) {
+            tr_keep[w] = P.tr[t];
+            vk[w][0] = P.v[t][0];
+            vk[w][1] = P.v[t][1];
+            vk[w][2] = P.v[t][2];
+            const int before = (w == 0 ? 0 : (w == 1 ? below0 : below1)) +
+                               __builtin_popcountll(out[w] & ((1ull << lane) - 1));
+            to[w] = t - before;
+        }
+    }
+    __syncthreads();
+    double m = 0.0;
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+        if (keep[w]) {
+            P.tr[to[w]] = tr_keep[w];
+            P.v[to[w]][0] = vk[w][0];
+            P.v[to[w]][1] = vk[w][1];
+            P.v[to[w]][2] = vk[w][2];
+            m = fmax(m, vk[w][0] * vk[w][0] + vk[w][1] * vk[w][1] + vk[w][2] * vk[w][2]);
+        }
+    }
+    __syncthreads();
+    S.nt = base - nout;
+    S.r2 = wave_max_d(m);
+    return true;
+}
+
+// (d2, j) strictly before (e2, k)
+__device__ __forceinline__ bool key_less(double d2, int j, double e2, int k) {
+    return d2 < e2 || (d2 == e2 && j < k);
+}
+
+// Sorts one key per lane ascending across the wave (bitonic).
+__device__ __forceinline__ void wave_sort64(double &d2, int &j) {
+    const int lane = __lane_id();
+    for (int k = 2; k <= 64; k <<= 1)
+        for (int s = k >> 1; s > 0; s >>= 1) {
+            const double od = __shfl_xor(d2, s, 64);
+            const int oj = __shfl_xor(j, s, 64);
+            const bool up = (lane & k) == 0;
+            const bool lower = (lane & s) == 0;
+            const bool other_less = key_less(od, oj, d2, j);
+            // ascending block: the lower lane keeps the smaller key
+            const bool take = (up == lower) ? other_less : key_less(d2, j, od, oj);
+            if (take) {
+                d2 = od;
+                j = oj;
+            }
+        }
+}
+
+// Merges a chunk of candidates (one per lane, invalid = +inf) into the
+// sorted best list of kc <= kNear entries.
+__device__ void wmerge_near(WavePoly &P, int &kc, double d2, int j) {
+    const int lane = __lane_id();
+    wave_sort64(d2, j);
+    P.cd[lane] = d2;
+    P.cj[lane] = j;
+    double bd = 0.0;
+    int bj = 0;
+    const bool has = lane < kc;
+    if (has) {
+        bd = P.kd[lane];
+        bj = P.kj[lane];
+    }
+    __syncthreads();
+    // candidate lane: entries of the best list before it (binary search)
+    int lo = 0, hi = kc;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (key_less(P.kd[mid], P.kj[mid], d2, j)) lo = mid + 1;
+        else hi = mid;
+    }
+    const int cpos = lane + lo;
+    // best entry: candidates before it
+    int lo2 = 0, hi2 = 64;
+    if (has) {
+        while (lo2 < hi2) {
+            const int mid = (lo2 + hi2) >> 1;
+            if (key_less(P.cd[mid], P.cj[mid], bd, bj)) lo2 = mid + 1;
+            else hi2 = mid;
+        }
+    }
+    const int bpos = lane + lo2;
+    const int nvalid = __builtin_popcountll(__ballot(d2 < INFINITY));
+    __syncthreads();
+    if (has && bpos < kNear) {
+        P.kd[bpos] = bd;
+        P.kj[bpos] = bj;
+    }
+    if (d2 < INFINITY && cpos < kNear) {
+        P.kd[cpos] = d2;
+        P.kj[cpos] = j;
+    }
+    kc = min(kc + nvalid, kNear);
+    __syncthreads();
+}
+
+// Position q of the Chebyshev shell of radius k (k >= 1): the two z faces,
+// then the rings of the layers between.
+__device__ __forceinline__ void shell_pos(int k, int q, int &x, int &y, int &z) {
+    const int s = 2 * k + 1;
+    if (q < 2 * s * s) {
+        const int r = q % (s * s);
+        z = q < s * s ? -k : k;
+        x = r % s - k;
+        y = r / s - k;
+        return;
+    }
+    q -= 2 * s * s;
+    z = q / (8 * k) - k + 1;
+    const int r = q % (8 * k);
+    if (r < s) {
+        y = -k;
+        x = r - k;
+    } else if (r < 2 * s) {
+        y = k;
+        x = r - s - k;
+    } else {
+        const int r2 = r - 2 * s;
+        x = r2 < 2 * k - 1 ? -k : k;
+        y = r2 % (2 * k - 1) - k + 1;
+    }
+}
+__device__ __forceinline__ int shell_size(int k) { return k == 0 ? 1 : 2 * (2 * k + 1) * (2 * k + 1) + 8 * k * (2 * k - 1); }
+
+// One wavefront per cell.  lists (N, max_nb) int32 sorted ascending, counts
+// (N), flags (N): bit 0 the cell touches the box boundary, bit 1 capacity
+// overflow (the row is incomplete), bit 2 the cell is empty (hidden site).
+// lifted (N): the lifted coordinate L = |p|^2 - r^2 as the caller computed
+// it (benchmark.py:29-32 computes it in f32).  slack >= |L - (|p|^2 - r^2)|
+// bounds that rounding.
+__global__ __launch_bounds__(64) void power_cells_wave_kernel(const float *pts, const float *lifted, int64_t N,
+                                                              PowerGrid g, const int64_t *gstart, const int *items,
+                                                              const unsigned char *occ, float3 blo, float3 bhi,
+                                                              float rmax, float slack, int max_nb, int *lists,
+                                                              int *counts, int *flags) {
+    __shared__ WavePoly P;
+    WaveState S;
+    const int64_t i = blockIdx.x;
+    if (i >= N) return;
+    const int lane = __lane_id();
+    const d3 pi = dd(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+    const double Li = (double)lifted[i];
+    winit_box(P, S, dd(blo.x - pi.x, blo.y - pi.y, blo.z - pi.z), dd(bhi.x - pi.x, bhi.y - pi.y, bhi.z - pi.z));
+    const int ix = bucket_of(pts[3 * i], g.ox, g.cell, g.nx[0]);
+    const int iy = bucket_of(pts[3 * i + 1], g.oy, g.cell, g.ny[0]);
+    const int iz = bucket_of(pts[3 * i + 2], g.oz, g.cell, g.nz[0]);
+    const int kmax = max(g.nx[0], max(g.ny[0], g.nz[0]));
+    auto offset = [&](int j) {
+        return dd((double)pts[3 * j] - pi.x, (double)pts[3 * j + 1] - pi.y, (double)pts[3 * j + 2] - pi.z);
+    };
+    auto rhs_of = [&](int j, d3 dj) { return 0.5 * ((double)lifted[j] - Li) - ddot(pi, dj); };
+
+    // 1. the kNear nearest sites in the order (|d_j|^2, j), ring by ring
+    int kc = 0;
+    for (int k = 0; k <= kmax; ++k) {
+        const int ns = shell_size(k);
+        for (int q0 = 0; q0 < ns; q0 += 256) {
+            // sites per shell position of this segment, prefix in LDS
+            int c4 = 0, cq[4];
+            int64_t bq[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int q = q0 + 4 * lane + m;
+                cq[m] = 0;
+                bq[m] = 0;
+                if (q < ns) {
+                    int x = 0, y = 0, z = 0;
+                    if (k > 0) shell_pos(k, q, x, y, z);
+                    x += ix;
+                    y += iy;
+                    z += iz;
+                    if (x >= 0 && x < g.nx[0] && y >= 0 && y < g.ny[0] && z >= 0 && z < g.nz[0]) {
+                        bq[m] = ((int64_t)z * g.ny[0] + y) * g.nx[0] + x;
+                        cq[m] = (int)(gstart[bq[m] + 1] - gstart[bq[m]]);
+                    }
+                }
+                c4 += cq[m];
+            }
+            int tot = 0;
+            int run = wave_excl_scan(c4, tot);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                P.cnt[4 * lane + m] = run;  // first site of position 4 lane + m
+                run += cq[m];
+            }
+            __syncthreads();
+            for (int s0 = 0; s0 < tot; s0 += 64) {
+                const int s = s0 + lane;
+                double d2 = INFINITY;
+                int j = 0x7fffffff;
+                if (s < tot) {
+                    // the position holding site s: last prefix <= s
+                    int lo = 0, hi = 255;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (P.cnt[mid] <= s) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const int q = q0 + lo;
+                    int x = 0, y = 0, z = 0;
+                    if (k > 0) shell_pos(k, q, x, y, z);
+                    const int64_t b = ((int64_t)(z + iz) * g.ny[0] + (y + iy)) * g.nx[0] + (x + ix);
+                    const int jj = items[gstart[b] + (s - P.cnt[lo])];
+                    if (jj != i) {
+                        const d3 dj = offset(jj);
+                        const double e2 = ddot(dj, dj);
+                        if (e2 > 1e-24 && (kc < kNear || key_less(e2, jj, P.kd[kNear - 1], P.kj[kNear - 1]))) {
+                            d2 = e2;
+                            j = jj;
+                        }
+                    }
+                }
+                if (__any(d2 < INFINITY)) wmerge_near(P, kc, d2, j);
+            }
+            __syncthreads();
+        }
+        const double D = ((double)k - 0.01) * (double)g.cell;
+        if (kc == kNear && k >= 1 && D * D > P.kd[kNear - 1]) break;
+    }
+    // 2. clip by them nearest first; later planes mostly miss the cell's ball
+    for (int t = 0; t < kc && S.nt > 0 && !S.overflow; ++t) {
+        const int j = P.kj[t];
+        const d3 dj = offset(j);
+        const double rhs = rhs_of(j, dj);
+        if (!misses_ball(dj, rhs, S.r2)) wclip(P, S, dj, rhs, j);
+    }
+    // 3. certify the vertices.  With fewer than kNear other sites every site
+    // has been clipped; otherwise the sites not clipped lie at least
+    // sqrt(kd[kNear - 1]) from p_i, and a site that cuts vertex v lies within
+    // rho of v, rho^2 = |v|^2 + r_max^2 - r_i^2 (+ the lift's rounding).
+    const double rm2 = (double)rmax * (double)rmax + 2.0 * (double)slack;
+    const double ri2 = ddot(pi, pi) - Li;
+    const double dK = kc == kNear ? sqrt(P.kd[kNear - 1]) : 0.0;
+    const double kd_last = kc == kNear ? P.kd[kNear - 1] : 0.0;
+    const int kj_last = kc == kNear ? P.kj[kNear - 1] : 0;
+    for (int round = 0; kc == kNear && S.nt > 0 && !S.overflow; ++round) {
+        if (round == kMaxRounds) {
+            S.overflow = true;
+            break;
+        }
+        // every uncertified vertex: the cheap tests, else its ball's first cutting site
+        int found[3] = {-1, -1, -1};
+        bool any_open = false;
+        for (int w = 0; w < 3; ++w) {
+            const int t = 64 * w + lane;
+            if (t >= S.nt) continue;
+            const unsigned tr = P.tr[t];
+            if (tr >> 24) continue;
+            const d3 v = wv(P, t);
+            const double v2 = ddot(v, v);
+            const double rho2 = (v2 + rm2 - ri2) * (1.0 + 1e-9) + 1e-18;
+            bool cert = rho2 <= 0.0;
+            double rho = 0.0;
+            if (!cert) {
+                rho = sqrt(rho2);
+                cert = (sqrt(v2) + rho) * (1.0 + 1e-9) < dK;
+            }
+            if (!cert) {
+                int cut = -1;
+                ball_query(g, occ, gstart, items, dd(pi.x + v.x, pi.y + v.y, pi.z + v.z), rho, [&](int j) {
+                    if (j == i) return true;
+                    const d3 dj = offset(j);
+                    const double d2 = ddot(dj, dj);
+                    if (d2 <= 1e-24) return true;
+                    if (d2 < kd_last || (d2 == kd_last && j <= kj_last)) return true;  // clipped in 2.
+                    const double rhs = rhs_of(j, dj);
+                    if (!(ddot(dj, v) > rhs)) return true;
+                    for (int p = 0; p < S.np; ++p)
+                        if (P.nb[p] == j) return true;  // applied already: rounding only
+                    cut = j;
+                    return false;
+                });
+                cert = cut < 0;
+                found[w] = cut;
+            }
+            if (cert) P.tr[t] = tr | (1u << 24);
+            else any_open = true;
+        }
+        __syncthreads();
+        if (!__any(any_open)) break;
+        // clip by the sites found, lane by lane (the polytope changes between clips)
+        for (int w = 0; w < 3; ++w) {
+            unsigned long long pend = __ballot(found[w] >= 0);
+            while (pend && S.nt > 0 && !S.overflow) {
+                const int src = __ffsll((unsigned long long)pend) - 1;
+                pend &= pend - 1;
+                const int j = __shfl(found[w], src, 64);
+                bool applied = false;
+                for (int p = 0; p < S.np; ++p) applied |= P.nb[p] == j;
+                if (applied) continue;
+                const d3 dj = offset(j);
+                wclip(P, S, dj, rhs_of(j, dj), j);
+            }
+        }
+    }
+    int flag = S.overflow ? 2 : 0;
+    if (S.nt == 0) flag |= 4;
+    // neighbours: planes referenced by surviving triangles, ascending
+    unsigned long long used = 0;
+    for (int t = lane; t < S.nt; t += 64) {
+        const unsigned tr = P.tr[t];
+        used |= (1ull << tri_a(tr)) | (1ull << tri_b(tr)) | (1ull << tri_c(tr));
+    }
+    for (int off = 32; off > 0; off >>= 1) used |= __shfl_xor(used, off, 64);
+    const bool mine = lane < S.np && ((used >> lane) & 1);
+    const int nbv = mine ? P.nb[lane] : kBoxPlane;
+    if (__any(mine && nbv == kBoxPlane)) flag |= 1;
+    const bool val = mine && nbv != kBoxPlane;
+    const unsigned long long valm = __ballot(val);
+    int rank = 0;
+    for (unsigned long long m = valm; m;) {
+        const int src = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        rank += __shfl(nbv, src, 64) < nbv ? 1 : 0;
+    }
+    const int c = __builtin_popcountll(valm);
+    if (val && rank < max_nb) lists[i * (int64_t)max_nb + rank] = nbv;
+    if (lane == 0) {
+        counts[i] = c < max_nb ? c : max_nb;
+        flags[i] = flag | (c > max_nb ? 2 : 0);
+    }
 }
 
 __global__ __launch_bounds__(256) void pgrid_count_kernel(const float *pts, int64_t N, PowerGrid g, int *gcount,
@@ -682,7 +1140,7 @@ extern "C" int pf_power_cells(const float *points, const float *lifted, int64_t 
                        grid_start);
     hipLaunchKernelGGL(pgrid_scatter_kernel, dim3(nb), dim3(256), 0, st, num_cells, cell_key, grid_start, grid_fill,
                        grid_items);
-    hipLaunchKernelGGL(power_cells_kernel, dim3((unsigned)((num_cells + 63) / 64)), dim3(64), 0, st, points, lifted,
+    hipLaunchKernelGGL(power_cells_wave_kernel, dim3((unsigned)num_cells), dim3(64), 0, st, points, lifted,
                        num_cells, g, grid_start, grid_items, occupancy, make_float3(box[0], box[1], box[2]),
                        make_float3(box[3], box[4], box[5]), r_max, lift_slack, max_nb, lists, counts, flags);
     return hipGetLastError() == hipSuccess ? PF_OK : PF_ERR_HIP;

@@ -45,6 +45,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     if (p.out.depth) p.out.depth[tl.ray] = r.depth;
     if (p.out.visits) p.out.visits[tl.ray] = r.visits;
     if (p.out.seq_hash) p.out.seq_hash[tl.ray] = r.hash;
+    if (p.out.ray_status) p.out.ray_status[tl.ray] = r.status;
     if (p.out.alg_bytes)
         p.out.alg_bytes[tl.ray] = r.bytes + 12 + (p.out.log_t ? 4 : 0) + (p.out.depth ? 4 : 0) + (REC ? 4 : 0);
     if (r.status && p.out.status) atomicOr(p.out.status, r.status);

@@ -82,6 +82,7 @@ class PfWalkOut(ctypes.Structure):
         ("status", c_void_p),
         ("alg_bytes", c_void_p),
         ("record", ctypes.POINTER(PfWalkRecord)),
+        ("ray_status", c_void_p),
     ]
 
 

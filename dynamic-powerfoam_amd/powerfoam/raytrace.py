@@ -231,7 +231,7 @@ def _expect(t, shape, name, dtype=torch.float32, device=None):
 
 def launch_walk_forward(scene, cams_dev, width, height, transmittance_threshold=1e-3, depth_quantile=0.5,
                         max_steps=DEFAULT_MAX_STEPS, want_log_t=True, want_depth=True, want_sequences=False,
-                        seq_cap=0, want_bytes=False, record_cap=0):
+                        seq_cap=0, want_bytes=False, record_cap=0, want_ray_status=False):
     """One walk launch over a batch of cameras.  record_cap > 0 also writes
     the record of contributing segments that launch_walk_backward(record=)
     differentiates without replaying the walk (out["record"])."""
@@ -261,6 +261,8 @@ def launch_walk_forward(scene, cams_dev, width, height, transmittance_threshold=
     out.status = _lib.ptr(status)
     alg_bytes = torch.zeros(B, height, width, dtype=torch.int64, device=dev) if want_bytes else None
     out.alg_bytes = _lib.ptr(alg_bytes)
+    ray_status = torch.empty(B, height, width, dtype=torch.int32, device=dev) if want_ray_status else None
+    out.ray_status = _lib.ptr(ray_status)
     record = rec = None
     if record_cap > 0:
         record = new_record(B, width, height, record_cap, dev)
@@ -271,7 +273,7 @@ def launch_walk_forward(scene, cams_dev, width, height, transmittance_threshold=
                              ctypes.byref(out), _lib.stream_ptr())
     _lib.check(rc, "pf_walk_forward")
     return dict(color=color, log_t=log_t, depth=depth, visits=visits, seq_hash=seq_hash, seq=seq, status=status,
-                alg_bytes=alg_bytes, record=record)
+                alg_bytes=alg_bytes, record=record, ray_status=ray_status)
 
 
 def launch_walk_backward(scene, cams_dev, width, height, color, log_t, grad_color, grad_log_t,

@@ -138,6 +138,9 @@ typedef struct pf_walk_out {
                              visit sphere+CSR bounds+[n,sigma] and per face
                              index+face; per shaded segment K texels     */
     const pf_walk_record *record;  /* NULL, or the record to write        */
+    int32_t  *ray_status; /* (B,H,W)   each ray's PF_STATUS_* bits (may be
+                             NULL), e.g. to count the rays the loop guard
+                             stopped                                        */
 } pf_walk_out;
 
 /* Gradients produced by the walk backward; all f32, accumulated (+=) so the
@@ -564,7 +567,7 @@ int pf_serve_step_grouped(const pf_serve_scene *scenes,
 /* The power diagram's adjacency (build_power_adjacency, benchmark.py:25-83,
  * which lifts the sites to 4-D for a Qhull hull) for the part of the
  * diagram inside box = {lo[3], hi[3]}: for each site i, the sites j whose
- * bisector face with i meets the box.  One thread per site clips the box by
+ * bisector face with i meets the box.  One wavefront per site clips the box by
  * the bisectors of its nearest sites in FP64, then certifies each vertex of
  * the cell with an empty-ball lookup in an occupancy pyramid over the grid
  * buckets.  grid (cell size and dims over the sites' bounds, dims <= 2^20

@@ -52,7 +52,7 @@ def load():
                                                           c_void_p, c_double, c_int32, c_int64]
         lib.orc_walk_forward.argtypes = [c_void_p, c_void_p, c_int32, c_float, c_float, c_int32, c_void_p, c_int64,
                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
-                                         c_int32]
+                                         c_int32, c_void_p]
         lib.orc_walk_backward.argtypes = [c_void_p, c_void_p, c_int32, c_float, c_int32] + [c_void_p] * 8 + [c_int32, c_int32]
         lib.orc_start_cell.argtypes = [c_void_p, c_void_p, c_int64, c_void_p]
         lib.orc_walk_faces.argtypes = [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int32]
@@ -171,10 +171,12 @@ def walk_forward(scene, cams, thr=1e-3, depth_q=0.5, max_steps=1 << 16, rays=Non
     visits = np.empty(shape, np.int32)
     hsh = np.empty(shape, np.uint64)
     seq = np.full(shape + (seq_cap,), -1, np.int32) if seq_cap > 0 else None
+    ray_status = np.empty(shape, np.int32)
     st = lib.orc_walk_forward(ctypes.byref(scene.c), _p(cams), B, thr, depth_q, max_steps, _p(rays),
                               0 if rays is None else rays.size, _p(color), _p(log_t), _p(depth), _p(visits), _p(hsh),
-                              _p(seq), seq_cap, nthreads or num_threads())
-    return dict(color=color, log_t=log_t, depth=depth, visits=visits, seq_hash=hsh, seq=seq, status=st)
+                              _p(seq), seq_cap, nthreads or num_threads(), _p(ray_status))
+    return dict(color=color, log_t=log_t, depth=depth, visits=visits, seq_hash=hsh, seq=seq, status=st,
+                ray_status=ray_status)
 
 
 def walk_backward(scene, cams, color, grad_color, grad_log_t=None, thr=1e-3, max_steps=1 << 16, exact=False,

@@ -727,7 +727,7 @@ int32_t orc_start_cell(const float *points, const float *radii, int64_t N, const
 // rays: optional list of ray ids (b*H*W + i*W + j) to trace; NULL = all.
 int orc_walk_forward(const SceneIn *scene, const CamIn *cams, int32_t B, float thr, float depth_q, int32_t max_steps,
                      const int64_t *rays, int64_t n_rays, float *color, float *log_t, float *depth, int32_t *visits,
-                     uint64_t *hash, int32_t *seq, int32_t seq_cap, int32_t nthreads) {
+                     uint64_t *hash, int32_t *seq, int32_t seq_cap, int32_t nthreads, int32_t *ray_status) {
     const SceneR<float> s = view_f32(*scene);
     const FaceLoadF32 fl{scene};
     const int W = cams[0].width, H = cams[0].height;
@@ -751,6 +751,7 @@ int orc_walk_forward(const SceneIn *scene, const CamIn *cams, int32_t B, float t
         if (depth) depth[o] = r.depth;
         if (visits) visits[o] = r.visits;
         if (hash) hash[o] = r.hash;
+        if (ray_status) ray_status[o] = r.status;
         status |= r.status;
     }
     return status;

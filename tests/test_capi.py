@@ -41,12 +41,12 @@ def test_struct_layouts_match_header():
 
     # the same sizes are static_asserts in csrc/capi.hip, so the library
     # and this binding agree or the library does not build
-    sizes = dict(PfCamera=48, PfScene=104, PfWalkOut=80, PfWalkRecord=40, PfGrads=48, PfServeScene=120,
+    sizes = dict(PfCamera=48, PfScene=104, PfWalkOut=88, PfWalkRecord=40, PfGrads=48, PfServeScene=120,
                  PfServeRequest=168, PfCechGrid=32)
     for name, size in sizes.items():
         assert ctypes.sizeof(getattr(_lib, name)) == size, name
     src = open(os.path.join(ROOT, "dynamic-powerfoam_amd", "csrc", "capi.hip")).read()
-    for cname, size in (("pf_camera", 48), ("pf_scene", 104), ("pf_walk_out", 80), ("pf_walk_record", 40),
+    for cname, size in (("pf_camera", 48), ("pf_scene", 104), ("pf_walk_out", 88), ("pf_walk_record", 40),
                         ("pf_grads", 48), ("pf_serve_scene", 120), ("pf_serve_request", 168),
                         ("pf_cech_grid", 32)):
         assert f"sizeof({cname}) == {size}" in src, cname

@@ -290,10 +290,13 @@ def test_loop_guard_matches_cpu(hit_b):
     pts, rad = t(S.spheres[:, :3]), t(S.spheres[:, 3])
     gs = rt.WalkScene(pts, rad, t(S.nsigmas[:, 3]), t(S.nsigmas[:, :3]), t(S.texel_sites), t(S.texel_rgb),
                       t(S.texel_height), t(S.adjacency), t(S.offsets), t(S.faces))
-    g = rt.launch_walk_forward(gs, t(words), 3, 3, want_sequences=True, seq_cap=64)
+    g = rt.launch_walk_forward(gs, t(words), 3, 3, want_sequences=True, seq_cap=64, want_ray_status=True)
     torch.cuda.synchronize()
     c = orc.walk_forward(S, words, seq_cap=64)
     assert int(g["status"].item()) == c["status"] == (0 if hit_b else 4)
+    # each ray's own bits: the rays the guard stopped, counted alike
+    assert np.array_equal(g["ray_status"].cpu().numpy(), c["ray_status"])
+    assert np.bitwise_or.reduce(c["ray_status"].reshape(-1)) == c["status"]
     assert np.array_equal(g["visits"].cpu().numpy(), c["visits"])
     assert np.array_equal(g["seq"].cpu().numpy(), c["seq"])
     assert np.array_equal(g["log_t"].cpu().numpy(), c["log_t"])
