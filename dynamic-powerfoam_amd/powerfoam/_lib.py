@@ -117,7 +117,7 @@ class PfVisOut(ctypes.Structure):
 
 
 class PfCechGrid(ctypes.Structure):
-    _fields_ = [("origin", c_float * 3), ("cell", c_float), ("dims", c_int32 * 3), ("reserved", c_int32)]
+    _fields_ = [("origin", c_float * 3), ("cell", c_float), ("dims", c_int32 * 3), ("levels", c_int32)]
 
 
 class PfServeScene(ctypes.Structure):
@@ -206,10 +206,12 @@ _SIGS = {
         [c_void_p, c_void_p, c_int64, ctypes.POINTER(PfCechGrid), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
          c_void_p, c_void_p, c_void_p],
     ),
+    "pf_cech_grid_size": (c_int64, [ctypes.POINTER(PfCechGrid)]),
+    "pf_cech_write_workspace_bytes": (c_int64, [c_int64, c_int64]),
     "pf_cech_write": (
         c_int32,
-        [c_void_p, c_void_p, c_int64, ctypes.POINTER(PfCechGrid), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-         c_void_p],
+        [c_void_p, c_void_p, c_int64, ctypes.POINTER(PfCechGrid), c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
+         c_void_p, c_void_p, c_int64, c_void_p],
     ),
     "pf_power_workspace_bytes": (c_int64, [ctypes.POINTER(PfCechGrid)]),
     "pf_morton_codes": (c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),

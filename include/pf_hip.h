@@ -431,19 +431,28 @@ int pf_raster_visualize(const pf_scene *scene, const pf_camera *cam,
 
 /* ---- Čech complex (AABBTree) ------------------------------------------- */
 
-/* Uniform grid for the overlap search: cell >= 2 max radius.             */
+/* Grid for the overlap search, in levels: level l has cells of size
+ * cell * 2^l from origin and dims ((dims[k] - 1) >> l) + 1; a sphere is
+ * listed in the finest level whose cells are at least its diameter, so the
+ * last level's cell must be >= 2 r_max.  levels 0 or 1: one uniform grid
+ * (cell >= 2 r_max).  Buckets: pf_cech_grid_size of them.  (pf_power_cells
+ * reads origin, cell and dims only.)                                      */
 typedef struct pf_cech_grid {
     float origin[3];
     float cell;
     int32_t dims[3];
-    int32_t reserved;
+    int32_t levels;
 } pf_cech_grid;
 
-/* Bounds of the recovered sphere centres and the largest radius
- * (AABBTree.update, bvh.py:76-94): bounds[0..2] min, [3..5] max, [6] r_max
- * (device floats).  workspace: 7 int32 (device).                           */
+/* Bounds of the recovered sphere centres, the largest and the mean radius
+ * (AABBTree.update, bvh.py:76-94): bounds[0..2] min, [3..5] max, [6] r_max,
+ * [7] mean radius (device floats, 8).  workspace: 7 int32 (device).       */
 int pf_cech_bounds(const float *centers, const float *radii, int64_t num_cells,
                    int32_t *workspace, float *bounds, void *stream);
+
+/* Buckets of the grid's levels together (G below), or -1 when the grid is
+ * out of range (more than 16 levels, or 2^31 buckets or more).           */
+int64_t pf_cech_grid_size(const pf_cech_grid *grid);
 
 /* count_adjacent (bvh.py:7-33) plus its cumsum (:109): buckets the spheres
  * into the grid, counts each sphere's overlapping neighbours and scans the
@@ -457,11 +466,17 @@ int pf_cech_count(const float *centers, const float *radii, int64_t num_cells,
                   int64_t *row_offsets, void *stream);
 
 /* write_adjacent (bvh.py:36-69): fills each CSR row (sorted ascending) and
- * the int32 offsets (N+1).  adjacency holds row_offsets[N] entries.        */
+ * the int32 offsets (N+1).  adjacency holds num_edges = row_offsets[N]
+ * entries; workspace: pf_cech_write_workspace_bytes(num_edges, num_cells)
+ * bytes, any contents (the unsorted rows and the row sort's storage).
+ * -1: sizes out of range.                                                 */
+int64_t pf_cech_write_workspace_bytes(int64_t num_edges, int64_t num_cells);
 int pf_cech_write(const float *centers, const float *radii, int64_t num_cells,
                   const pf_cech_grid *grid, const int64_t *grid_start,
                   const int32_t *grid_items, const int64_t *row_offsets,
-                  int32_t *adjacency, int32_t *adjacency_offsets, void *stream);
+                  int64_t num_edges, int32_t *adjacency,
+                  int32_t *adjacency_offsets, void *workspace,
+                  int64_t workspace_bytes, void *stream);
 
 /* ---- serving ------------------------------------------------------------ */
 /* No reference counterpart: the reference has no serving path (its closest

@@ -55,6 +55,30 @@ def test_cech_textured_and_isolated_spheres(scene_textured):
     assert np.array_equal(off.cpu().numpy(), wo) and np.array_equal(adj.cpu().numpy(), wa)
 
 
+def test_cech_heavy_tailed_radii_equals_brute_force():
+    """60,000 cells with Pareto radii (most small, a few up to 200 times the
+    median, as trained scenes have): the grid takes several levels, level 0
+    sized by the mean radius, and the edge set still equals brute force.
+    One grid sized by the largest radius would hold ~10^3 times more
+    candidates per query."""
+    import time
+
+    rng = np.random.default_rng(11)
+    sc = workloads.gmm_scene(60_000, seed=4)
+    rad = (0.004 * (1.0 + rng.pareto(1.5, 60_000))).clip(max=0.8).astype(np.float32)
+    tree = AABBTree(DEV)
+    tree.update(t(sc["points"]), t(rad))
+    assert tree.grid.levels >= 5
+    t0 = time.perf_counter()
+    adj, off = tree.build_cech_complex()
+    torch.cuda.synchronize()
+    took = time.perf_counter() - t0
+    wa, wo = orc.cech(sc["points"], rad)
+    assert np.array_equal(off.cpu().numpy(), wo)
+    assert np.array_equal(adj.cpu().numpy(), wa)
+    assert took < 5.0, took
+
+
 def test_cech_60k_cells_equals_brute_force():
     """A clustered 60,000-cell scene (radii tripled, so rows are not empty):
     the grid and row offsets go through the multi-chunk scan (15 chunks of

@@ -85,3 +85,4 @@ def test_launcher_runs_entry_point(scene):
     np.testing.assert_allclose(b[0:3], c.min(0).values.cpu().numpy(), rtol=1e-5, atol=1e-6)
     np.testing.assert_allclose(b[3:6], c.max(0).values.cpu().numpy(), rtol=1e-5, atol=1e-6)
     np.testing.assert_allclose(b[6], float(r.max()), rtol=1e-5)
+    np.testing.assert_allclose(b[7], float(r.mean()), rtol=1e-4)  # mean radius
