@@ -27,7 +27,7 @@
 // The box B is the sites' bounding box with a margin: the result is every
 // face that meets B.  Cells whose final polytope touches B's boundary are
 // flagged; the faces lying wholly outside B are faces of the power diagram of
-// the flagged cells alone, which power_hull_kernel completes.
+// the flagged cells alone, which power_hull_wave_kernel completes.
 #include "pf_device.h"
 #include "pf_hip.h"
 
@@ -81,23 +81,6 @@ __host__ inline int64_t pyramid(PowerGrid &g) {
     return bytes;
 }
 
-struct Poly {
-    Plane pl[kMaxPlanes];
-    int nb[kMaxPlanes];
-    unsigned char tr[kMaxTris][3];
-    unsigned char cert[kMaxTris];  // vertex certified against every site
-    d3 v[kMaxTris];
-    double r2;  // max |v|^2 over the vertices: the cell lies in the ball of radius sqrt(r2)
-    int np, nt;
-    bool overflow;
-};
-
-__device__ __forceinline__ void update_radius(Poly &P) {
-    double r2 = 0.0;
-    for (int t = 0; t < P.nt; ++t) r2 = fmax(r2, ddot(P.v[t], P.v[t]));
-    P.r2 = r2;
-}
-
 // True when n . y <= d holds on the whole ball |y|^2 <= r2, so that the
 // plane cannot cut the cell.  The relative margin covers the rounding of the
 // vertex dot products clip() compares.
@@ -105,155 +88,10 @@ __device__ __forceinline__ bool misses_ball(d3 n, double d, double r2) {
     return d > 0.0 && d * d > r2 * ddot(n, n) * (1.0 + 1e-9);
 }
 
-__device__ __forceinline__ d3 vertex_of(const Poly &P, int a, int b, int c) {
-    const Plane &A = P.pl[a], &B = P.pl[b], &C = P.pl[c];
-    const d3 bc = dcross(B.n, C.n), ca = dcross(C.n, A.n), ab = dcross(A.n, B.n);
-    const double det = ddot(A.n, bc);
-    const double s = 1.0 / det;
-    return dd((A.d * bc.x + B.d * ca.x + C.d * ab.x) * s, (A.d * bc.y + B.d * ca.y + C.d * ab.y) * s,
-              (A.d * bc.z + B.d * ca.z + C.d * ab.z) * s);
-}
-
-// The box [lo, hi] (local coordinates) as an octahedron of 8 oriented
-// triangles: corner (sx, sy, sz) is planes (X, Y, Z), counter-clockwise
-// seen from outside when sx sy sz > 0, swapped otherwise.
-__device__ __forceinline__ void init_box(Poly &P, d3 lo, d3 hi) {
-    P.pl[0] = Plane{dd(1, 0, 0), hi.x};
-    P.pl[1] = Plane{dd(-1, 0, 0), -lo.x};
-    P.pl[2] = Plane{dd(0, 1, 0), hi.y};
-    P.pl[3] = Plane{dd(0, -1, 0), -lo.y};
-    P.pl[4] = Plane{dd(0, 0, 1), hi.z};
-    P.pl[5] = Plane{dd(0, 0, -1), -lo.z};
-    for (int k = 0; k < 6; ++k) P.nb[k] = kBoxPlane;
-    P.np = 6;
-    P.nt = 0;
-    for (int sx = 0; sx < 2; ++sx)
-        for (int sy = 0; sy < 2; ++sy)
-            for (int sz = 0; sz < 2; ++sz) {
-                const int X = sx ? 1 : 0, Y = sy ? 3 : 2, Z = sz ? 5 : 4;
-                const bool positive = ((sx + sy + sz) & 1) == 0;
-                const int t = P.nt++;
-                P.tr[t][0] = (unsigned char)(positive ? X : Y);
-                P.tr[t][1] = (unsigned char)(positive ? Y : X);
-                P.tr[t][2] = (unsigned char)Z;
-                P.cert[t] = 0;
-                P.v[t] = dd(sx ? lo.x : hi.x, sy ? lo.y : hi.y, sz ? lo.z : hi.z);
-            }
-    P.overflow = false;
-    update_radius(P);
-}
-
-// Drops planes no triangle references (they were clipped away).
-__device__ __forceinline__ void compact_planes(Poly &P) {
-    unsigned long long used = 0;
-    for (int t = 0; t < P.nt; ++t)
-        for (int k = 0; k < 3; ++k) used |= 1ull << P.tr[t][k];
-    unsigned char remap[kMaxPlanes];
-    int n = 0;
-    for (int p = 0; p < P.np; ++p) {
-        if ((used >> p) & 1) {
-            P.pl[n] = P.pl[p];
-            P.nb[n] = P.nb[p];
-            remap[p] = (unsigned char)n++;
-        }
-    }
-    P.np = n;
-    for (int t = 0; t < P.nt; ++t)
-        for (int k = 0; k < 3; ++k) P.tr[t][k] = remap[P.tr[t][k]];
-}
-
-// Clips by n . y <= d.  Returns true when the plane cut the cell.  Kept
-// triangles keep their order and certification; new ones are appended.
-__device__ bool clip(Poly &P, d3 n, double d, int nbr) {
-    unsigned long long out[(kMaxTris + 63) / 64] = {0, 0, 0};
-    int nout = 0;
-    for (int t = 0; t < P.nt; ++t)
-        if (ddot(n, P.v[t]) > d) {
-            out[t >> 6] |= 1ull << (t & 63);
-            ++nout;
-        }
-    if (nout == 0) return false;
-    if (nout == P.nt) {  // nothing of the cell survives: a hidden site
-        P.nt = 0;
-        return true;
-    }
-    if (P.np == kMaxPlanes) compact_planes(P);
-    if (P.np == kMaxPlanes) {
-        P.overflow = true;
-        return false;
-    }
-    const int p = P.np++;
-    P.pl[p] = Plane{n, d};
-    P.nb[p] = nbr;
-    // boundary edges: an edge a->b of a dropped triangle whose twin b->a
-    // belongs to a kept triangle; each gives the new triangle (a, b, p)
-    int nt = P.nt;
-    for (int t = 0; t < P.nt; ++t) {
-        if (!((out[t >> 6] >> (t & 63)) & 1)) continue;
-        for (int k = 0; k < 3; ++k) {
-            const int a = P.tr[t][k], b = P.tr[t][(k + 1) % 3];
-            bool twin_dropped = false;
-            for (int u = 0; u < P.nt && !twin_dropped; ++u) {
-                if (u == t || !((out[u >> 6] >> (u & 63)) & 1)) continue;
-                for (int m = 0; m < 3; ++m)
-                    if (P.tr[u][m] == b && P.tr[u][(m + 1) % 3] == a) twin_dropped = true;
-            }
-            if (twin_dropped) continue;
-            if (nt == kMaxTris) {
-                P.overflow = true;
-                return false;
-            }
-            P.tr[nt][0] = (unsigned char)a;
-            P.tr[nt][1] = (unsigned char)b;
-            P.tr[nt][2] = (unsigned char)p;
-            P.cert[nt] = 0;
-            P.v[nt] = vertex_of(P, a, b, p);
-            ++nt;
-        }
-    }
-    // drop the clipped triangles (keep order of the rest)
-    int w = 0;
-    for (int t = 0; t < nt; ++t) {
-        const bool dropped = t < P.nt && ((out[t >> 6] >> (t & 63)) & 1);
-        if (dropped) continue;
-        if (w != t) {
-            P.tr[w][0] = P.tr[t][0];
-            P.tr[w][1] = P.tr[t][1];
-            P.tr[w][2] = P.tr[t][2];
-            P.cert[w] = P.cert[t];
-            P.v[w] = P.v[t];
-        }
-        ++w;
-    }
-    P.nt = w;
-    update_radius(P);
-    return true;
-}
-
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 __device__ __forceinline__ int bucket_of(float p, float o, float cell, int n) {
     return clampi((int)floorf((p - o) / cell), 0, n - 1);
-}
-
-// Calls fn(j) for every site j bucketed in the shell of Chebyshev radius k
-// around bucket (ix, iy, iz).  Returns false as soon as fn does.
-template <class F>
-__device__ __forceinline__ bool for_shell(const PowerGrid &g, const int64_t *gstart, const int *items, int ix, int iy,
-                                          int iz, int k, F &&fn) {
-    for (int z = max(iz - k, 0); z <= min(iz + k, g.nz[0] - 1); ++z)
-        for (int y = max(iy - k, 0); y <= min(iy + k, g.ny[0] - 1); ++y) {
-            const bool yz_edge = abs(z - iz) == k || abs(y - iy) == k;
-            // inside the shell only x = ix +- k belongs to it
-            const int step = yz_edge ? 1 : 2 * k;
-            for (int x = ix - k; x <= ix + k; x += step) {
-                if (x < 0 || x >= g.nx[0]) continue;
-                const int64_t b = ((int64_t)z * g.ny[0] + y) * g.nx[0] + x;
-                for (int64_t q = gstart[b]; q < gstart[b + 1]; ++q)
-                    if (!fn(items[q])) return false;
-            }
-        }
-    return true;
 }
 
 // Calls fn(j) for every site j in a bucket that meets the ball |x - c| <= rho
@@ -311,78 +149,6 @@ __device__ bool ball_query(const PowerGrid &g, const unsigned char *occ, const i
                 }
             }
     return true;
-}
-
-// Writes the planes selected by keep into row (sorted ascending, box planes
-// left out).  Returns the flag bits 0 (a box plane was selected) and 1 (row
-// capacity).
-__device__ int write_row(const Poly &P, unsigned long long keep, int max_nb, int *row, int &count) {
-    int flag = 0, c = 0;
-    for (int p = 0; p < P.np; ++p) {
-        if (!((keep >> p) & 1)) continue;
-        if (P.nb[p] == kBoxPlane) {
-            flag |= 1;
-            continue;
-        }
-        if (c == max_nb) {
-            flag |= 2;
-            break;
-        }
-        // insertion into the sorted row
-        int v = P.nb[p], w = c++;
-        while (w > 0 && row[w - 1] > v) {
-            row[w] = row[w - 1];
-            --w;
-        }
-        row[w] = v;
-    }
-    count = c;
-    return flag;
-}
-
-// Faces outside the box.  Every point outside box B is owned by a site whose
-// cell meets B's boundary, so the diagram outside B is that of those sites
-// (hull) alone.  One thread per hull site h clips a far box by the
-// bisectors of all other hull sites and keeps the planes whose face has a
-// vertex outside B (a convex face with every vertex inside B lies inside
-// B).  Output as power_cells_wave_kernel (lists (nh, max_nb) of global ids).
-__global__ __launch_bounds__(64) void power_hull_kernel(const float *pts, const float *lifted, const int *hull,
-                                                        int nh, float3 blo, float3 bhi, float far, int max_nb,
-                                                        int *lists, int *counts, int *flags) {
-    const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= nh) return;
-    const int i = hull[h];
-    Poly P;
-    const d3 pi = dd(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
-    const double Li = (double)lifted[i];
-    init_box(P, dd(-far, -far, -far), dd(far, far, far));
-    // list neighbours first, alternating sides: the hull list is in cell
-    // order, and with cells in Morton order neighbours in the list are near
-    // in space, so the cell shrinks early and later planes miss its ball
-    for (int k = 1; k < nh && P.nt > 0 && !P.overflow; ++k) {
-        const int step = (k + 1) >> 1;
-        int q = (k & 1) ? h + step : h - step;
-        if (q >= nh) q -= nh;
-        if (q < 0) q += nh;
-        const int j = hull[q];
-        if (j == i) continue;
-        const d3 dj = dd((double)pts[3 * j] - pi.x, (double)pts[3 * j + 1] - pi.y, (double)pts[3 * j + 2] - pi.z);
-        if (ddot(dj, dj) <= 1e-24) continue;
-        const double rhs = 0.5 * ((double)lifted[j] - Li) - ddot(pi, dj);
-        if (misses_ball(dj, rhs, P.r2)) continue;
-        clip(P, dj, rhs, j);
-    }
-    const d3 lo = dd(blo.x - pi.x, blo.y - pi.y, blo.z - pi.z), hi = dd(bhi.x - pi.x, bhi.y - pi.y, bhi.z - pi.z);
-    unsigned long long outside = 0;  // planes with a face vertex outside B
-    for (int t = 0; t < P.nt; ++t) {
-        const d3 v = P.v[t];
-        if (v.x < lo.x || v.y < lo.y || v.z < lo.z || v.x > hi.x || v.y > hi.y || v.z > hi.z)
-            for (int m = 0; m < 3; ++m) outside |= 1ull << P.tr[t][m];
-    }
-    int c = 0;
-    const int flag = (P.overflow ? 2 : 0) | (write_row(P, outside, max_nb, lists + (int64_t)h * max_nb, c) & 2);
-    counts[h] = c;
-    flags[h] = flag;
 }
 
 // ------------------------------------------------------ one wave per cell --
@@ -756,6 +522,29 @@ __device__ __forceinline__ void shell_pos(int k, int q, int &x, int &y, int &z) 
 }
 __device__ __forceinline__ int shell_size(int k) { return k == 0 ? 1 : 2 * (2 * k + 1) * (2 * k + 1) + 8 * k * (2 * k - 1); }
 
+// Writes the neighbours of the planes in mask (box planes left out) to row,
+// ascending, at most max_nb of them, and their number to *count.  Returns
+// flag bits 0 (a box plane is in mask) and 1 (more than max_nb).
+__device__ int wwrite_row(const WavePoly &P, const WaveState &S, unsigned long long mask, int max_nb, int *row,
+                          int *count) {
+    const int lane = __lane_id();
+    const bool mine = lane < S.np && ((mask >> lane) & 1);
+    const int nbv = mine ? P.nb[lane] : kBoxPlane;
+    int flag = __any(mine && nbv == kBoxPlane) ? 1 : 0;
+    const bool val = mine && nbv != kBoxPlane;
+    const unsigned long long valm = __ballot(val);
+    int rank = 0;
+    for (unsigned long long m = valm; m;) {
+        const int src = __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        rank += __shfl(nbv, src, 64) < nbv ? 1 : 0;
+    }
+    const int c = __builtin_popcountll(valm);
+    if (val && rank < max_nb) row[rank] = nbv;
+    if (lane == 0) *count = c < max_nb ? c : max_nb;
+    return flag | (c > max_nb ? 2 : 0);
+}
+
 // One wavefront per cell.  lists (N, max_nb) int32 sorted ascending, counts
 // (N), flags (N): bit 0 the cell touches the box boundary, bit 1 capacity
 // overflow (the row is incomplete), bit 2 the cell is empty (hidden site).
@@ -936,23 +725,74 @@ __global__ __launch_bounds__(64) void power_cells_wave_kernel(const float *pts, 
         used |= (1ull << tri_a(tr)) | (1ull << tri_b(tr)) | (1ull << tri_c(tr));
     }
     for (int off = 32; off > 0; off >>= 1) used |= __shfl_xor(used, off, 64);
-    const bool mine = lane < S.np && ((used >> lane) & 1);
-    const int nbv = mine ? P.nb[lane] : kBoxPlane;
-    if (__any(mine && nbv == kBoxPlane)) flag |= 1;
-    const bool val = mine && nbv != kBoxPlane;
-    const unsigned long long valm = __ballot(val);
-    int rank = 0;
-    for (unsigned long long m = valm; m;) {
-        const int src = __ffsll((unsigned long long)m) - 1;
-        m &= m - 1;
-        rank += __shfl(nbv, src, 64) < nbv ? 1 : 0;
+    flag |= wwrite_row(P, S, used, max_nb, lists + i * (int64_t)max_nb, counts + i);
+    if (lane == 0) flags[i] = flag;
+}
+
+// Faces outside the box.  Every point outside box B is owned by a site whose
+// cell meets B's boundary, so the diagram outside B is that of those sites
+// (hull) alone.  One wavefront per hull site h clips a far box by the
+// bisectors of the other hull sites and keeps the planes whose face has a
+// vertex outside B (a convex face with every vertex inside B lies inside
+// B).  The list neighbours come first, alternating sides: the hull list is
+// in cell order, and with cells in Morton order neighbours in the list are
+// near in space, so the cell shrinks early and later planes miss its ball.
+// The lanes test 64 candidates at a time against the cell's ball; those
+// that may cut are tested again with the current ball (which only shrinks:
+// a clip's new vertices lie on the old edges) and clipped in the list
+// order.  Output as power_cells_wave_kernel (lists (nh, max_nb) of global
+// ids).
+__global__ __launch_bounds__(64) void power_hull_wave_kernel(const float *pts, const float *lifted, const int *hull,
+                                                             int nh, float3 blo, float3 bhi, float far, int max_nb,
+                                                             int *lists, int *counts, int *flags) {
+    __shared__ WavePoly P;
+    WaveState S;
+    const int h = blockIdx.x;
+    if (h >= nh) return;
+    const int lane = __lane_id();
+    const int i = hull[h];
+    const d3 pi = dd(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+    const double Li = (double)lifted[i];
+    winit_box(P, S, dd(-far, -far, -far), dd(far, far, far));
+    for (int k0 = 1; k0 < nh && S.nt > 0 && !S.overflow; k0 += 64) {
+        const int k = k0 + lane;
+        d3 dj = dd(0, 0, 0);
+        double rhs = 0.0;
+        int j = -1;
+        bool maybe = false;
+        if (k < nh) {
+            const int step = (k + 1) >> 1;
+            int q = (k & 1) ? h + step : h - step;
+            if (q >= nh) q -= nh;
+            if (q < 0) q += nh;
+            j = hull[q];
+            dj = dd((double)pts[3 * j] - pi.x, (double)pts[3 * j + 1] - pi.y, (double)pts[3 * j + 2] - pi.z);
+            rhs = 0.5 * ((double)lifted[j] - Li) - ddot(pi, dj);
+            maybe = j != i && ddot(dj, dj) > 1e-24 && !misses_ball(dj, rhs, S.r2);
+        }
+        unsigned long long pend = __ballot(maybe);
+        while (pend && S.nt > 0 && !S.overflow) {
+            const int src = __ffsll((unsigned long long)pend) - 1;
+            pend &= pend - 1;
+            const d3 n = dd(__shfl(dj.x, src, 64), __shfl(dj.y, src, 64), __shfl(dj.z, src, 64));
+            const double d = __shfl(rhs, src, 64);
+            const int js = __shfl(j, src, 64);
+            if (misses_ball(n, d, S.r2)) continue;
+            wclip(P, S, n, d, js);
+        }
     }
-    const int c = __builtin_popcountll(valm);
-    if (val && rank < max_nb) lists[i * (int64_t)max_nb + rank] = nbv;
-    if (lane == 0) {
-        counts[i] = c < max_nb ? c : max_nb;
-        flags[i] = flag | (c > max_nb ? 2 : 0);
+    const d3 lo = dd(blo.x - pi.x, blo.y - pi.y, blo.z - pi.z), hi = dd(bhi.x - pi.x, bhi.y - pi.y, bhi.z - pi.z);
+    unsigned long long outside = 0;  // planes with a face vertex outside B
+    for (int t = lane; t < S.nt; t += 64) {
+        const d3 v = wv(P, t);
+        if (v.x < lo.x || v.y < lo.y || v.z < lo.z || v.x > hi.x || v.y > hi.y || v.z > hi.z) {
+            const unsigned tr = P.tr[t];
+            outside |= (1ull << tri_a(tr)) | (1ull << tri_b(tr)) | (1ull << tri_c(tr));
+        }
     }
+    for (int off = 32; off > 0; off >>= 1) outside |= __shfl_xor(outside, off, 64);
+    const int f = wwrite_row(P, S, outside, max_nb, lists + (int64_t)h * max_nb, counts + h);
+    if (lane == 0) flags[h] = (S.overflow ? 2 : 0) | (f & 2);
 }
 
 __global__ __launch_bounds__(256) void pgrid_count_kernel(const float *pts, int64_t N, PowerGrid g, int *gcount,
@@ -1154,8 +994,8 @@ extern "C" int pf_power_hull(const float *points, const float *lifted, const int
         return PF_ERR_ARG;
     if (max_nb <= 0 || max_nb > kMaxPlanes) return PF_ERR_ARG;
     if (num_hull == 0) return PF_OK;
-    hipLaunchKernelGGL(power_hull_kernel, dim3((unsigned)((num_hull + 63) / 64)), dim3(64), 0, (hipStream_t)stream,
-                       points, lifted, hull, num_hull, make_float3(box[0], box[1], box[2]),
+    hipLaunchKernelGGL(power_hull_wave_kernel, dim3((unsigned)num_hull), dim3(64), 0, (hipStream_t)stream, points,
+                       lifted, hull, num_hull, make_float3(box[0], box[1], box[2]),
                        make_float3(box[3], box[4], box[5]), far, max_nb, lists, counts, flags);
     return hipGetLastError() == hipSuccess ? PF_OK : PF_ERR_HIP;
 }
