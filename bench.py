@@ -422,7 +422,9 @@ def c3_bench(args):
            "walk_bwd_ms": round(bwd_ms, 3), "walk_fwd_alg_bytes": fwd_bytes, "walk_bwd_alg_bytes": bwd_bytes,
            "record": segs, "status": int(f["status"].item()) | int(b["status"].item()),
            "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4)}}
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "note": "algorithmic bytes count every visit's reads of a scene small enough to stay in "
+                                "the 256 MB Infinity Cache across the 16 cameras, so frac can pass 1"}}
     log(f"16 cameras: step {dt * 1e3:.2f} ms = {out['mrays_per_s']} Mrays/s; fwd {fwd_ms:.2f} ms, bwd {bwd_ms:.2f} ms; "
         f"record {segs}")
     return out
