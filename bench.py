@@ -67,6 +67,9 @@ def main():
                     help="segments per ray the server records for forward+backward tiles (0 = replay)")
     ap.add_argument("--serve-oldest-share", type=float, default=1.0,
                     help="share of a step's capacity the oldest active request may take")
+    ap.add_argument("--face-layout", choices=("packed", "f16"), default="f16",
+                    help="walk faces as 12-byte records with the neighbour (packed) or the (E, 4) fp16 array "
+                         "with the adjacency beside it (f16)")
     ap.add_argument("--record-cap", type=int, default=128,
                     help="contributing segments per ray the forward records for the backward (0 = replay)")
     ap.add_argument("--backward", choices=("grouped", "recorded", "replay"), default="grouped",
@@ -143,7 +146,7 @@ def main():
     K = sites.shape[1]
     texel_rgb = torch.empty(1, args.cells, K, 3, dtype=torch.float32, device=dev)
     scene = rt.WalkScene(pts, rad, t(sc["density"]), t(sc["normals"]), sites, texel_rgb[0], t(sc["texel_height"]),
-                         adj_d, off_d, faces)
+                         adj_d, off_d, faces, pack_faces=args.face_layout == "packed")
     target = t(target_np)
     W, H = args.width, args.height
     grads = dict(spheres=torch.zeros(args.cells, 4, device=dev), nsigmas=torch.zeros(args.cells, 4, device=dev),
@@ -298,7 +301,7 @@ def main():
             "width": W,
             "height": H,
             "rays_per_step": rays,
-            "face_storage": "f16",
+            "face_storage": "f16" if args.face_layout == "f16" else "f16 packed with the neighbour",
             "cell_order": args.cell_order,
             "adjacency": args.adjacency,
             "adjacency_build_s": round(adj_seconds, 2),
@@ -364,7 +367,7 @@ def c3_bench(args):
     texel_rgb = torch.empty(B, n, K, 3, device="cuda")
     shading.sh_forward(coeffs, 3, sites, cams_dev, out=texel_rgb)
     scene = rt.WalkScene(pts, rad, t(sc["density"]), t(sc["normals"]), sites, texel_rgb, t(sc["texel_height"]), adj,
-                         off, faces)
+                         off, faces, pack_faces=args.face_layout == "packed")
     cap = rt.RayTracer(None, "cuda").record_cap_for(B, W, H)
     workspace = rt.grouped_workspace(n, B, W, H, cap, "cuda")
     target = torch.rand(B, H, W, 3, generator=torch.Generator().manual_seed(args.seed + 8)).to("cuda")
@@ -518,7 +521,7 @@ def raster_bench(sc, pts, rad, texel_rgb, target, cam, args):
 
 def build_served_scene(n, seed):
     """A resident GMM scene of n cells (as the benchmark's: SH degree 3,
-    Morton-ordered cells, GPU power-diagram adjacency, packed walk faces)."""
+    Morton-ordered cells, GPU power-diagram adjacency, fp16 walk faces)."""
     import numpy as np
     import torch
 

@@ -58,6 +58,7 @@ __device__ __forceinline__ RayFwd walk_wave_fwd(const SceneDev<A> &s, const RGB 
     o = mk(__int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(o.x))),
            __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(o.y))),
            __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(o.z))));
+    const RowReader<F> rows(s.faces, s.E);
     ws.dir[3 * lane] = d.x;
     ws.dir[3 * lane + 1] = d.y;
     ws.dir[3 * lane + 2] = d.z;
@@ -128,16 +129,18 @@ __device__ __forceinline__ RayFwd walk_wave_fwd(const SceneDev<A> &s, const RGB 
                 bytes_since += (uint32_t)(kVisitBytes + (uint64_t)(e1 - e0) * kFaceBytes);
                 int exit_e = -1;
                 float pt_far = 1e10f;
-                scan_row<F>(s.faces, s.E, e0, e1, [&](int e, f3 fn, float fw) {
+                auto face = [&](int e, f3 fn, float fw, bool valid) {
                     float t, dp;
                     ray_pface_diff(o, d, fn, fw, t, dp);
-                    if (dp >= 0.0f && t < pt_far) { exit_e = e; pt_far = t; }
+                    const bool leave = valid && dp >= 0.0f, enter = valid && dp < 0.0f;
+                    if (leave && t < pt_far) { exit_e = e; pt_far = t; }
                     if constexpr (REC) {
-                        if (dp < 0.0f && t > tn) tn_id = e - e0;
+                        if (enter && t > tn) tn_id = e - e0;
                     }
-                    tf = dp >= 0.0f ? fmin_(t, tf) : tf;
-                    tn = dp < 0.0f ? fmax_(t, tn) : tn;
-                });
+                    tf = leave ? fmin_(t, tf) : tf;
+                    tn = enter ? fmax_(t, tn) : tn;
+                };
+                rows.scan(e0, e1, face);
                 const int next = exit_e >= 0 ? face_neighbour<F>(s.faces, s.adj, exit_e) : kNoCell;
                 if (sh.hit && !(tn > tf) && !(sigma < 1e-3f)) {
                     if constexpr (REC) {

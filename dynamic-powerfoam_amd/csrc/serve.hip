@@ -44,7 +44,7 @@ __device__ __forceinline__ int find_request(const pf_serve_request *reqs, int n,
 template <typename RGB>
 __device__ __forceinline__ int serve_ray(const SceneDev<float> &s, const RGB &colors, const pf_serve_request &rq,
                                          const Cam &cam, f3 d, int64_t pix, float thr, int max_steps) {
-    const RayFwd r = walk_ray_fwd<float, PackedF16, 8>(s, colors, cam.eye, d, cam.start, thr, 0.5f, max_steps,
+    const RayFwd r = walk_ray_fwd<float, _Float16, 8>(s, colors, cam.eye, d, cam.start, thr, 0.5f, max_steps,
                                                        nullptr, 0);
     rq.color[3 * pix] = r.rgb.x;
     rq.color[3 * pix + 1] = r.rgb.y;
@@ -74,10 +74,10 @@ __device__ __forceinline__ int serve_ray_bwd(const SceneDev<float> &s, const RGB
     f3 gc = mk(0, 0, 0);
     if (valid) {
         if (rec.cap > 0)
-            r = walk_ray_fwd<float, PackedF16, 8, RGB, true>(s, colors, cam.eye, d, cam.start, thr, 0.5f, max_steps,
+            r = walk_ray_fwd<float, _Float16, 8, RGB, true>(s, colors, cam.eye, d, cam.start, thr, 0.5f, max_steps,
                                                              nullptr, 0, rec);
         else
-            r = walk_ray_fwd<float, PackedF16, 8>(s, colors, cam.eye, d, cam.start, thr, 0.5f, max_steps, nullptr,
+            r = walk_ray_fwd<float, _Float16, 8>(s, colors, cam.eye, d, cam.start, thr, 0.5f, max_steps, nullptr,
                                                   0);
         rq.color[3 * pix] = r.rgb.x;
         rq.color[3 * pix + 1] = r.rgb.y;
@@ -96,17 +96,17 @@ __device__ __forceinline__ int serve_ray_bwd(const SceneDev<float> &s, const RGB
     bool replay = live;
     if (rec.cap > 0) {
         const bool held = r.segs <= rec.cap;
-        st |= walk_rec_bwd<float, PackedF16, 8, RGB, ServeCache>(s, colors, cam.eye, d, rec.rec, rec.slot, rec.S,
+        st |= walk_rec_bwd<float, _Float16, 8, RGB, ServeCache>(s, colors, cam.eye, d, rec.rec, rec.slot, rec.S,
                                                                  r.segs, r.rgb, gc, 0.0f, G, false, bytes,
                                                                  live && held, cache);
         replay = live && !held;
     }
-    return st | walk_ray_bwd<float, PackedF16, 8, RGB, ServeCache>(s, colors, cam.eye, d, cam.start, thr, max_steps,
+    return st | walk_ray_bwd<float, _Float16, 8, RGB, ServeCache>(s, colors, cam.eye, d, cam.start, thr, max_steps,
                                                                    r.rgb, gc, 0.0f, G, false, bytes, replay, cache);
 }
 
-// Served scenes use f32 attributes, K = 8 and packed walk faces
-// (PF_F16_PACKED).  The table lists forward entries first: tiles [0, desc[2]) are forward
+// Served scenes use f32 attributes, K = 8 and (E, 4) fp16 walk faces
+// (PF_F16), read in pairs.  The table lists forward entries first: tiles [0, desc[2]) are forward
 // only and [desc[2], desc[1]) forward+backward.  BWD selects the range, so
 // forward-only tiles run in a kernel without the backward's registers.
 // rec_segs / rec_cap: the forward+backward tiles' record workspace (one
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     const f3 d = ray_dir(cam, l.pi, l.pj);
     const SceneDev<float> s = scene_dev<float>(sc.scene);
     const float *rgb = rq.texel_rgb ? rq.texel_rgb : reinterpret_cast<const float *>(sc.scene.texel_rgb);
-    const RayFwd r = walk_wave_fwd<float, PackedF16, MemRGB<float>, true>(
+    const RayFwd r = walk_wave_fwd<float, _Float16, MemRGB<float>, true>(
         s, MemRGB<float>{rgb}, cam.eye, d, cam.start, q.thr, 0.5f, q.max_steps, nullptr, 0,
         SegRecord{q.segs, slot, q.S, q.cap, q.part}, shade[threadIdx.x >> 6]);
     q.counts[slot] = r.segs;
@@ -448,7 +448,7 @@ __device__ __forceinline__ void serve_replay(const ServeGroup &q, int64_t begin,
             const SceneDev<float> s = scene_dev<float>(sc.scene);
             const float *rgb = rq.texel_rgb ? rq.texel_rgb : reinterpret_cast<const float *>(sc.scene.texel_rgb);
             uint64_t bytes = 0;
-            const int st = walk_ray_bwd<float, PackedF16, 8>(s, MemRGB<float>{rgb}, cam.eye, d, cam.start, q.thr,
+            const int st = walk_ray_bwd<float, _Float16, 8>(s, MemRGB<float>{rgb}, cam.eye, d, cam.start, q.thr,
                                                              q.max_steps, C, gc, 0.0f, request_grads(rq),
                                                              q.exact_texel, bytes);
             if (st && q.status) atomicOr(q.status, st);
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int64_t lo = w * per;
     const int64_t hi = lo + per < M ? lo + per : M;
     if (lo >= hi) return;
-    grouped_range<float, PackedF16>(ServeGroupCtx{q.exact_texel, q.scenes, q.reqs}, q.ent, lo, hi,
+    grouped_range<float, _Float16>(ServeGroupCtx{q.exact_texel, q.scenes, q.reqs}, q.ent, lo, hi,
                                     rows + (threadIdx.x >> 6) * kGroupRows * kGroupPitch);
 }
 

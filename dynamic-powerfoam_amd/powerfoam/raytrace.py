@@ -40,7 +40,7 @@ class WalkScene:
     """
 
     def __init__(self, points, radii, density, normals, texel_sites, texel_rgb, texel_height, adjacency,
-                 adjacency_offsets, faces, attr_dtype=torch.float32, pack_faces=True):
+                 adjacency_offsets, faces, attr_dtype=torch.float32, pack_faces=False):
         dev = points.device
         if dev.type != "cuda":
             raise ValueError("Only CUDA tensors are supported as inputs")
@@ -76,7 +76,9 @@ class WalkScene:
         elif self.faces.dtype in (torch.float16, torch.float32) and self.faces.shape == (E, 4):
             face_code = _lib.dtype_code(self.faces.dtype)
             if pack_faces and face_code == _lib.PF_F16:
-                # same values, one 12-byte record per face: fewer gathers
+                # same values, one 12-byte record per face with its
+                # neighbour (the unpacked fp16 array is read two faces per
+                # load, which measured faster in the walk)
                 self.faces = pack_walk_faces(self.faces, self.adjacency)
                 face_code = _lib.PF_F16_PACKED
         else:

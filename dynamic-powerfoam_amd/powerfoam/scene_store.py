@@ -1,7 +1,7 @@
 """On-disk format for served scenes.
 
 One safetensors file holds every array a resident scene reads, the
-power-diagram CSR and the packed walk faces included, so that bringing a
+power-diagram CSR and the fp16 walk faces included, so that bringing a
 scene into a server is a file read and an upload: no Qhull lift or GPU
 adjacency build, no face precompute (benchmark.py:25-83, :295-302 run once,
 when the file is written).  safetensors stores raw little-endian arrays and
@@ -18,17 +18,19 @@ texel_sites        float32      (N, K, 3)
 texel_height       float32      (N, K)
 adjacency          int32        (E,)
 adjacency_offsets  int32        (N + 1,)
-walk_faces         int32        (E, 3)  packed records (four fp16 face
-                                        values, then the neighbour)
+walk_faces         float16      (E, 4)  the walk faces (benchmark.py:295-302)
 sh                 float32      (N, 16, 3)  when colour is "sh3"
 texel_rgb          float32      (N, K, 3)   when colour is "rgb"
 =================  ===========  ====================================
 
-Header metadata: ``format`` = ``powerfoam-scene/1``, ``colour``.
+Header metadata: ``format`` = ``powerfoam-scene/2``, ``colour``.  Files of
+``powerfoam-scene/1`` (walk_faces as int32 (E, 3) records: the four fp16
+values, then the neighbour) are read too, their faces unpacked on reading.
 """
 import torch
 
-FORMAT = "powerfoam-scene/1"
+FORMAT = "powerfoam-scene/2"
+FORMAT_PACKED = "powerfoam-scene/1"
 
 _COMMON = {
     "spheres": (torch.float32, lambda N, K, E: (N, 4)),
@@ -37,7 +39,7 @@ _COMMON = {
     "texel_height": (torch.float32, lambda N, K, E: (N, K)),
     "adjacency": (torch.int32, lambda N, K, E: (E,)),
     "adjacency_offsets": (torch.int32, lambda N, K, E: (N + 1,)),
-    "walk_faces": (torch.int32, lambda N, K, E: (E, 3)),
+    "walk_faces": (torch.float16, lambda N, K, E: (E, 4)),
 }
 _COLOUR = {
     "sh3": ("sh", torch.float32, lambda N, K, E: (N, 16, 3)),
@@ -45,14 +47,17 @@ _COLOUR = {
 }
 
 
-def check_scene_arrays(arrays, colour):
+def check_scene_arrays(arrays, colour, packed=False):
     """Raise ValueError unless arrays hold exactly the format's tensors with
-    consistent shapes and dtypes (CSR offsets included)."""
+    consistent shapes and dtypes (CSR offsets included); packed: the
+    powerfoam-scene/1 walk faces."""
     if colour not in _COLOUR:
         raise ValueError(f"unknown colour mode {colour!r}")
     name, dtype, shape = _COLOUR[colour]
     spec = dict(_COMMON)
     spec[name] = (dtype, shape)
+    if packed:
+        spec["walk_faces"] = (torch.int32, lambda N, K, E: (E, 3))
     missing = sorted(set(spec) - set(arrays))
     extra = sorted(set(arrays) - set(spec))
     if missing or extra:
@@ -70,8 +75,8 @@ def check_scene_arrays(arrays, colour):
     adj = arrays["adjacency"]
     if E and (int(adj.min()) < 0 or int(adj.max()) >= N):
         raise ValueError("adjacency holds an index out of range")
-    # the packed records carry the neighbour in their last word
-    if not torch.equal(arrays["walk_faces"][:, 2], adj):
+    # packed records carry the neighbour in their last word
+    if packed and not torch.equal(arrays["walk_faces"][:, 2], adj):
         raise ValueError("walk_faces neighbours disagree with the adjacency")
 
 
@@ -90,11 +95,14 @@ def read_scene_arrays(path, device="cpu"):
 
     with safe_open(str(path), framework="pt", device="cpu") as f:
         meta = f.metadata() or {}
-        if meta.get("format") != FORMAT:
+        if meta.get("format") not in (FORMAT, FORMAT_PACKED):
             raise ValueError(f"{path}: not a {FORMAT} file (format {meta.get('format')!r})")
+        packed = meta.get("format") == FORMAT_PACKED
         colour = meta.get("colour")
         arrays = {k: f.get_tensor(k) for k in f.keys()}
-    check_scene_arrays(arrays, colour)
+    check_scene_arrays(arrays, colour, packed)
+    if packed:
+        arrays["walk_faces"] = arrays["walk_faces"][:, :2].contiguous().view(torch.float16).view(-1, 4)
     if device != "cpu":
         arrays = {k: v.to(device, non_blocking=True) for k, v in arrays.items()}
     return arrays, colour

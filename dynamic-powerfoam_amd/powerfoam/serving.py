@@ -129,7 +129,7 @@ def server_streams(device, n):
 
 
 class ResidentScene:
-    """Device arrays of one scene (f32 attributes, packed f16 walk faces, K = 8)
+    """Device arrays of one scene (f32 attributes, (E, 4) fp16 walk faces, K = 8)
     plus its colour: fixed texel_rgb, degree-3 SH coefficients (turned into
     texel colours by a per-request pre-pass, or evaluated inside the walk),
     or spherical-Voronoi lobes ``sv = (axes (D, N*K, 3), values (D, N*K, 3),
@@ -141,15 +141,13 @@ class ResidentScene:
         if sh is None and texel_rgb is None and sv is None:
             raise ValueError("a scene needs SH coefficients, spherical-Voronoi lobes or texel_rgb")
         K = texel_sites.shape[1]
-        if K != 8 or faces.dtype not in (torch.float16, torch.int32):
-            raise ValueError("served scenes use K = 8 texels and fp16 (or packed) walk faces")
+        if K != 8 or faces.dtype != torch.float16 or faces.dim() != 2 or faces.shape[1] != 4:
+            raise ValueError("served scenes use K = 8 texels and (E, 4) fp16 walk faces")
         # with SH colour the kernel never reads texel_rgb; texel_sites (same
         # shape) stands in so the scene struct holds a valid pointer
         rgb = texel_rgb if texel_rgb is not None else texel_sites
         self.walk = WalkScene(points, radii, density, normals, texel_sites, rgb, texel_height, adjacency,
-                              adjacency_offsets, faces)
-        if self.walk.c.face_dtype != _lib.PF_F16_PACKED:  # the step kernel reads packed faces
-            raise ValueError("served scenes need walk faces that pack to 12-byte records")
+                              adjacency_offsets, faces, pack_faces=False)
         self.points = points.detach().float().contiguous()
         self.radii = radii.detach().float().contiguous()
         self.sh = sh.detach().float().contiguous() if sh is not None else None

@@ -485,7 +485,7 @@ int pf_cech_write(const float *centers, const float *radii, int64_t num_cells,
 /* A scene resident on the device: f32 attributes, f16 walk faces, K = 8.
  * sh: (N,16,3) degree-3 coefficients evaluated per texel site inside the
  * walk, or NULL to use scene.texel_rgb.                                    */
-/* A served scene: f32 attributes, K = 8 texels, PF_F16_PACKED walk faces;
+/* A served scene: f32 attributes, K = 8 texels, PF_F16 (E,4) walk faces;
  * sh (N,16,3) f32 degree-3 coefficients or NULL (then texel_rgb).        */
 typedef struct pf_serve_scene {
     pf_scene scene;

@@ -149,7 +149,7 @@ def _c3_batch(f16):
 
 
 def test_262k_cells_16_cameras_per_camera_colours_fp16_faces():
-    """The differentiation configuration as benchmarked: fp16 (packed) walk
+    """The differentiation configuration as benchmarked: fp16 walk
     faces, per-camera colours, the reference's texel approximations; the
     grouped backward (per-camera colour buckets), the ray-by-ray recorded
     backward and the replay against the oracle on every parameter."""

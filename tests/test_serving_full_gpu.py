@@ -34,7 +34,7 @@ KEYS = ("spheres", "nsigmas", "texel_sites", "texel_rgb", "texel_height")
 
 def _served_scene(n, seed):
     """A resident scene as the serving benchmark builds it: Morton-ordered
-    cells, the GPU power-diagram adjacency, packed fp16 walk faces."""
+    cells, the GPU power-diagram adjacency, (E, 4) fp16 walk faces."""
     sc = workloads.gmm_scene(n, seed=seed)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
     pts = t(sc["points"])

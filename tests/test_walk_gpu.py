@@ -20,7 +20,7 @@ TOL = 1e-4
 DEV = "cuda"
 
 
-def _gpu_scene(sc, attr=torch.float32, face_f16=True, packed=True):
+def _gpu_scene(sc, attr=torch.float32, face_f16=True, packed=False):
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
     pts, rad = t(sc["points"]), t(sc["radii"])
     adj, off = t(sc["adjacency"]).int(), t(sc["offsets"]).int()
@@ -46,7 +46,7 @@ def _cams(sc, cams):
     return pack_cameras(cams, starts)
 
 
-def _compare_forward(sc, cams, attr=torch.float32, face_f16=True, seq_cap=64, packed=True):
+def _compare_forward(sc, cams, attr=torch.float32, face_f16=True, seq_cap=64, packed=False):
     gs, faces = _gpu_scene(sc, attr, face_f16, packed)
     words = _cams(sc, cams)
     cams_dev = torch.from_numpy(words).to(DEV)
