@@ -127,20 +127,13 @@ __device__ __forceinline__ RayFwd walk_wave_fwd(const SceneDev<A> &s, const RGB 
                 const int e0 = s.off[prim];
                 const int e1 = s.off[prim + 1];
                 bytes_since += (uint32_t)(kVisitBytes + (uint64_t)(e1 - e0) * kFaceBytes);
-                int exit_e = -1;
-                float pt_far = 1e10f;
-                auto face = [&](int e, f3 fn, float fw, bool valid) {
-                    float t, dp;
-                    ray_pface_diff(o, d, fn, fw, t, dp);
-                    const bool leave = valid && dp >= 0.0f, enter = valid && dp < 0.0f;
-                    if (leave && t < pt_far) { exit_e = e; pt_far = t; }
-                    if constexpr (REC) {
-                        if (enter && t > tn) tn_id = e - e0;
-                    }
-                    tf = leave ? fmin_(t, tf) : tf;
-                    tn = enter ? fmax_(t, tn) : tn;
-                };
-                rows.scan(e0, e1, face);
+                FaceFold ff;
+                rows.scan(e0, e1, [&](int e, f3 fn, float fw, bool valid) { ff.add(e, o, d, fn, fw, valid); });
+                const int exit_e = ff.exit_face();
+                const float pt_far = ff.exit_t();
+                if constexpr (REC) tn_id = ff.near_id(tn, e0, tn_id);
+                tf = ff.t_far(tf);
+                tn = ff.t_near(tn);
                 const int next = exit_e >= 0 ? face_neighbour<F>(s.faces, s.adj, exit_e) : kNoCell;
                 if (sh.hit && !(tn > tf) && !(sigma < 1e-3f)) {
                     if constexpr (REC) {
