@@ -112,10 +112,10 @@ __device__ __forceinline__ void scan_row(const void *faces, int64_t E, int e0, i
 // fp16 faces (E, 4) in pairs: one 16-byte buffer load brings two faces,
 // which halves the loads a row costs.  The walk's forward is bound by the
 // vector memory pipeline's requests (each lane of a gather touches its own
-// cache line), not by bytes: 12.65 instead of 13.45 ms at 1080p.  Pairs are
-// aligned to even face indices, so a load never straddles a line; faces of
-// a pair outside the row are passed to fn as not valid.  A batch is two
-// pairs, and the next batch is in flight while the current one is
+// cache line), not by bytes: 12.65 instead of 13.45 ms at 1080p.  Pairs
+// start at the row's first face; a pair that runs past the row's end
+// brings the next row's first face, passed to fn as not valid.  A batch is
+// two pairs, and the next batch is in flight while the current one is
 // evaluated (two register buffers, used alternately), so that a row waits
 // for about one load latency instead of one per batch.  A pair past the
 // row's end is not read: its offset lies past the array, where a buffer
@@ -139,8 +139,8 @@ __device__ __forceinline__ void for_f16_face_pairs(__amdgpu_buffer_rsrc_t rs, in
         // issued here: not sunk past the loop's exit test to their first use
         __asm__ volatile("" ::: "memory");
     };
-    const uint32_t deg = (uint32_t)(e1 - e0);
     auto eval = [&](const uint4(&buf)[2], int f) {
+        const int rem = e1 - f;  // faces of the row from f on
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint4 q = buf[k >> 1];
@@ -149,13 +149,12 @@ __device__ __forceinline__ void for_f16_face_pairs(__amdgpu_buffer_rsrc_t rs, in
                             (float)__builtin_bit_cast(_Float16, (uint16_t)(lo >> 16)),
                             (float)__builtin_bit_cast(_Float16, (uint16_t)(hi & 0xffffu)));
             const float w = (float)__builtin_bit_cast(_Float16, (uint16_t)(hi >> 16));
-            fn(f + k, n, w, (uint32_t)(f + k - e0) < deg);
+            fn(f + k, n, w, k < rem);
         }
     };
     if (e0 >= e1) return;
-    const int f0 = e0 & ~1;
-    fetch(a, f0);
-    for (int f = f0;; f += 8) {
+    fetch(a, e0);
+    for (int f = e0;; f += 8) {
         fetch(b, f + 4);
         eval(a, f);
         if (f + 4 >= e1) break;
