@@ -87,6 +87,40 @@ def test_walk_faces_bit_equal(scene4k, f16):
     assert np.array_equal(got, want)
 
 
+def _permuted(sc, order):
+    """The scene with cell order[i] renumbered i, its power-diagram CSR
+    rebuilt in the new order."""
+    from powerfoam.adjacency import build_power_adjacency
+
+    out = {k: np.ascontiguousarray(sc[k][order]) for k in ("points", "radii", "density", "normals",
+                                                           "texel_sites", "texel_rgb", "texel_height")}
+    adj, off = build_power_adjacency(torch.from_numpy(out["points"]), torch.from_numpy(out["radii"]))
+    out["adjacency"], out["offsets"] = adj.numpy(), off.numpy()
+    return out
+
+
+@pytest.mark.parametrize("packed", [False, True])
+def test_rows_at_the_end_of_the_face_array(scene4k, packed):
+    """The cells a frame visits most, renumbered last: their CSR rows sit at
+    the end of the face array, where the paired face loads run past the
+    array's end (and read zeros) rather than into the next row.  Same bar
+    as every walk: sequences bit-exact against the oracle."""
+    sc = scene4k
+    cams = [workloads.front_camera(64, 48)]
+    _, faces = _gpu_scene(sc)
+    c = orc.walk_forward(_cpu_scene(sc, faces), _cams(sc, cams), seq_cap=256)
+    seq = c["seq"][c["seq"] >= 0]
+    hits = np.bincount(seq, minlength=len(sc["radii"]))
+    order = np.argsort(hits, kind="stable")  # most visited last
+    assert hits[order[-1]] > 0 and hits[order[0]] == 0
+    ps = _permuted(sc, order)
+    E = ps["adjacency"].shape[0]
+    last = int(np.searchsorted(ps["offsets"], E - 1, side="right")) - 1  # the cell owning the last row
+    g, cp = _compare_forward(ps, cams, packed=packed, seq_cap=256)
+    visited = cp["seq"][cp["seq"] >= 0]
+    assert last == len(ps["radii"]) - 1 and last in set(visited.tolist())
+
+
 def test_walk_uniform_4k_128px_bit_exact(scene4k):
     """The 4,096-cell, 128x128, 60 deg, seed-0 workload."""
     g, c = _compare_forward(scene4k, [workloads.front_camera(128, 128)])
