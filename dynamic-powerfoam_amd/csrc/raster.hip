@@ -437,6 +437,24 @@ __device__ __forceinline__ void stage_texels(float *tex, const RasterParams &p, 
     __syncthreads();
 }
 
+// DPP reads within rows of 16 lanes (a source off in EXEC reads 0).
+template <int CTRL, int ROWS = 0xF> __device__ __forceinline__ float dpp_row(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROWS, 0xF, false));
+}
+
+// Sum of x over the wavefront, in lane 63: row shifts by 1, 2, 4, 8, then
+// row_bcast:15 and row_bcast:31 (six DPP adds, where a butterfly of lane
+// shuffles takes six LDS-unit round trips).  Every lane must take part.
+__device__ __forceinline__ float wave_sum_last(float x) {
+    x += dpp_row<0x111>(x);
+    x += dpp_row<0x112>(x);
+    x += dpp_row<0x114>(x);
+    x += dpp_row<0x118>(x);
+    x += dpp_row<0x142, 0xA>(x);
+    x += dpp_row<0x143, 0xC>(x);
+    return x;
+}
+
 // ------------------------------------------------------------- forward --
 template <typename F>
 __global__ __launch_bounds__(64) void raster_fwd_kernel(RasterParams p) {
@@ -531,14 +549,12 @@ __global__ __launch_bounds__(64) void raster_fwd_kernel(RasterParams p) {
         }
         // per-cell sums over the tile's lanes, one atomic per wavefront
         if (__any(contrib != 0.0f)) {
-            float s = contrib;
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-            if (threadIdx.x == 0) unsafeAtomicAdd(p.contrib + prim, s);
+            const float s = wave_sum_last(contrib);
+            if (threadIdx.x == 63) unsafeAtomicAdd(p.contrib + prim, s);
         }
         if (p.point_err && __any(perr != 0.0f)) {
-            float s = perr;
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-            if (threadIdx.x == 0) unsafeAtomicAdd(p.perr + prim, s);
+            const float s = wave_sum_last(perr);
+            if (threadIdx.x == 63) unsafeAtomicAdd(p.perr + prim, s);
         }
     }
     if (!tc.oob) {
@@ -644,21 +660,61 @@ __device__ __forceinline__ void texel_bwd_raster(f3 o, f3 d, float tn_in, f3 c, 
 }
 
 // ------------------------------------------------------------ backward --
-// Gradient row per lane: [sphere 4 | nsigma 4 | texels 7K]; summed over the
-// wavefront through LDS (lane v sums column v), one atomic per value.
-constexpr int kRedCols = 8 + 7 * 8;  // K <= 8 on this path
-constexpr int kRedPitch = 33;
+// Sums each of 64 per-lane values over the wavefront and leaves column c's
+// total in lane c, by recursive halving: at each step a lane and its
+// partner keep complementary halves of the columns still held, each sends
+// the other half and adds what it receives.  Partners: lane ^ 32 and
+// lane ^ 16 through v_permlane32_swap / v_permlane16_swap (which also do the
+// sending), then DPP row_ror:8 (lane ^ 8 in a row), row_half_mirror (the
+// lanes of an 8-lane group in reverse order: partners differ in bit 2),
+// quad_perm xor 2 and xor 1.  141 VALU instructions and no LDS, for what a
+// transpose through LDS did with 64 writes, 128 reads and four barriers.
+// Every lane of the wavefront must take part.
+__device__ __forceinline__ float wave_column_sums(float (&v)[64]) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {  // v[i] -> column (lane & 32) + i
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[i]), __float_as_uint(v[32 + i]), false,
+                                                        false);
+        v[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {  // v[i] -> column (lane & 48) + i
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[i]), __float_as_uint(v[16 + i]), false,
+                                                        false);
+        v[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    auto half = [&](auto dpp, int n, bool upper) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (i >= n) break;
+            const float send = upper ? v[i] : v[n + i];
+            const float keep = upper ? v[n + i] : v[i];
+            v[i] = keep + dpp(send);
+        }
+    };
+    half([](float x) { return dpp_row<0x128>(x); }, 8, lane & 8);  // row_ror:8
+    half([](float x) { return dpp_row<0x141>(x); }, 4, lane & 4);  // row_half_mirror
+    half([](float x) { return dpp_row<0x4E>(x); }, 2, lane & 2);   // quad_perm [2,3,0,1]
+    half([](float x) { return dpp_row<0xB1>(x); }, 1, lane & 1);   // quad_perm [1,0,3,2]
+    return v[0];
+}
 
-// Three waves per SIMD: the uncapped kernel takes 206 VGPRs (two waves,
-// with LDS for three); capped at 168 it spills 9 and measured 10% faster
-// (1M-cell scene at 1920x1080: forward+backward 50.9 against 56.1 ms).
+// Gradient row per lane: [sphere 4 | nsigma 4 | texels 7K]; summed over the
+// wavefront by wave_column_sums (lane v gets column v), one atomic per value.
+constexpr int kRedCols = 8 + 7 * 8;  // K <= 8 on this path
+static_assert(kRedCols == 64, "one column per lane");
+
+// Three waves per SIMD: the uncapped kernel takes 206 VGPRs (two waves);
+// capped at 168 it measured 10% faster (1M-cell scene at 1920x1080:
+// forward+backward 50.9 against 56.1 ms).  Four waves (128 VGPRs, 46
+// spilled) measured 46.3 against 45.7 ms.
 template <typename F, int KT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void raster_bwd_kernel(RasterParams p) {
     using FT = typename RFace<F>::T;
     __shared__ PrimChunk pc;
     __shared__ FT sface[64];
     __shared__ float tex[7 * 8];
-    __shared__ float sred[64 * kRedPitch];
     __shared__ float sadj[64 * 4];
     const TileCtx tc = tile_ctx(p);
     if (tc.tile >= p.ntiles) return;
@@ -810,28 +866,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
             }
         }
         if (__any(act)) {
+            // columns past 8 + 7K hold zeros (red is cleared per entry)
             const int ncols = 8 + 7 * K;
-#pragma unroll
-            for (int c0 = 0; c0 < kRedCols; c0 += 32) {
-                if (c0 >= ncols) break;
-                __syncthreads();
-#pragma unroll
-                for (int v = 0; v < 32; ++v) sred[threadIdx.x * kRedPitch + v] = (c0 + v < ncols) ? red[c0 + v] : 0.0f;
-                __syncthreads();
-                if (threadIdx.x < 32 && c0 + (int)threadIdx.x < ncols) {
-                    float s = 0.0f;
-                    for (int l = 0; l < 64; ++l) s += sred[l * kRedPitch + threadIdx.x];
-                    const int col = c0 + threadIdx.x;
-                    if (s != 0.0f) {
-                        float *dst;
-                        if (col < 4) dst = p.gs_sph + 4 * (int64_t)prim + col;
-                        else if (col < 8) dst = p.gs_ns + 4 * (int64_t)prim + (col - 4);
-                        else if (col < 8 + 3 * K) dst = p.gs_sites + 3 * (int64_t)prim * K + (col - 8);
-                        else if (col < 8 + 6 * K) dst = p.gs_rgb + 3 * (int64_t)prim * K + (col - 8 - 3 * K);
-                        else dst = p.gs_h + (int64_t)prim * K + (col - 8 - 6 * K);
-                        unsafeAtomicAdd(dst, s);
-                    }
-                }
+            const float s = wave_column_sums(red);
+            const int col = threadIdx.x;
+            if (col < ncols && s != 0.0f) {
+                float *dst;
+                if (col < 4) dst = p.gs_sph + 4 * (int64_t)prim + col;
+                else if (col < 8) dst = p.gs_ns + 4 * (int64_t)prim + (col - 4);
+                else if (col < 8 + 3 * K) dst = p.gs_sites + 3 * (int64_t)prim * K + (col - 8);
+                else if (col < 8 + 6 * K) dst = p.gs_rgb + 3 * (int64_t)prim * K + (col - 8 - 3 * K);
+                else dst = p.gs_h + (int64_t)prim * K + (col - 8 - 6 * K);
+                unsafeAtomicAdd(dst, s);
             }
         }
     }
