@@ -202,41 +202,39 @@ struct RunScan {
     }
 };
 
-// Sum of x over all 64 lanes, in lane 63 (the in-row scan by row shifts,
-// then row_bcast:15 and row_bcast:31).  All 64 lanes must call it together.
-__device__ __forceinline__ float wave_total(float x) {
-    x += dpp_f<0x111, 0xF>(x);
-    x += dpp_f<0x112, 0xF>(x);
-    x += dpp_f<0x114, 0xF>(x);
-    x += dpp_f<0x118, 0xF>(x);
-    x += dpp_f<0x142, 0xA>(x);
-    x += dpp_f<0x143, 0xC>(x);
-    return x;
-}
-
 // Adds the lanes' neighbour-face terms (one side) to the neighbours'
 // sphere gradients: one round per distinct neighbour among the lanes, whose
 // terms are summed across the wave and added by one lane.  Neighbouring
 // rays through a cell leave it by few faces, so a wave has few distinct
 // neighbours per side and the atomics no longer pile up on one address.
+// Each round sums the four terms over the wave at once: halving by lane ^ 32
+// and lane ^ 16 (v_permlane32_swap / v_permlane16_swap) leaves term k's
+// partial sums in row k (16 lanes), which four DPP row shifts add up in the
+// row's last lane, and lanes 15, 31, 47 and 63 make the atomics (14 VALU
+// instructions a round, where four whole-wave sums took 28).
 __device__ __forceinline__ void add_face_terms(int nb, f3 dc, float dr, float *spheres) {
     uint64_t pend = __ballot(nb >= 0);
+    const int lane = __lane_id();
     while (pend) {
         const int lead = __ffsll((unsigned long long)pend) - 1;
         const int tgt = __builtin_amdgcn_readlane(nb, lead);
         const bool m = nb == tgt;
         pend &= ~__ballot(m);
-        const float sx = wave_total(m ? dc.x : 0.0f);
-        const float sy = wave_total(m ? dc.y : 0.0f);
-        const float sz = wave_total(m ? dc.z : 0.0f);
-        const float sr = wave_total(m ? dr : 0.0f);
-        if (__lane_id() == 63) {
-            float *ga = spheres + 4 * (int64_t)tgt;
-            if (sx != 0.0f) atomic_add(ga + 0, sx);
-            if (sy != 0.0f) atomic_add(ga + 1, sy);
-            if (sz != 0.0f) atomic_add(ga + 2, sz);
-            if (sr != 0.0f) atomic_add(ga + 3, sr);
-        }
+        const float x0 = m ? dc.x : 0.0f, x1 = m ? dc.y : 0.0f, x2 = m ? dc.z : 0.0f, x3 = m ? dr : 0.0f;
+        // lanes < 32 keep terms 0 and 1, lanes >= 32 terms 2 and 3
+        const auto p0 = __builtin_amdgcn_permlane32_swap(__float_as_uint(x0), __float_as_uint(x2), false, false);
+        const auto p1 = __builtin_amdgcn_permlane32_swap(__float_as_uint(x1), __float_as_uint(x3), false, false);
+        const float a0 = __uint_as_float(p0[0]) + __uint_as_float(p0[1]);
+        const float a1 = __uint_as_float(p1[0]) + __uint_as_float(p1[1]);
+        // rows 0 and 2 keep a0 (terms 0, 2), rows 1 and 3 a1 (terms 1, 3):
+        // row k now holds term k
+        const auto q = __builtin_amdgcn_permlane16_swap(__float_as_uint(a0), __float_as_uint(a1), false, false);
+        float t = __uint_as_float(q[0]) + __uint_as_float(q[1]);
+        t += dpp_f<0x111, 0xF>(t);
+        t += dpp_f<0x112, 0xF>(t);
+        t += dpp_f<0x114, 0xF>(t);
+        t += dpp_f<0x118, 0xF>(t);
+        if ((lane & 15) == 15 && t != 0.0f) atomic_add(spheres + 4 * (int64_t)tgt + (lane >> 4), t);
     }
 }
 
