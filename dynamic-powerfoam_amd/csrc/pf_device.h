@@ -168,12 +168,21 @@ __device__ __forceinline__ void ray_pface_diff(f3 o, f3 d, f3 diff, float pm_dif
     t = (pm_diff - dot(o, diff)) / dp;
 }
 
-// rendering_math.py:68-88 — raster face from the two spheres.
-__device__ __forceinline__ void ray_pface(f3 o, f3 d, f3 c, float r, f3 cj, float rj, float &t, float &dp) {
+// rendering_math.py:68-88 — raster face from the two spheres, in two
+// parts: what depends on the face and the ray origin only (the plane's
+// normal and off - o.n), and the ray's own quotient.  Rays of one camera
+// share the first part.
+__device__ __forceinline__ float4 pface_plane(f3 o, f3 c, float r, f3 cj, float rj) {
     const f3 n = cj - c;
     const float off = 0.5f * (dot(cj, cj) - dot(c, c) + r * r - rj * rj);
-    dp = dot(d, n);
-    t = (off - dot(o, n)) / dp;
+    return make_float4(n.x, n.y, n.z, off - dot(o, n));
+}
+__device__ __forceinline__ void pface_t(f3 d, float4 pl, float &t, float &dp) {
+    dp = dot(d, mk(pl.x, pl.y, pl.z));
+    t = pl.w / dp;
+}
+__device__ __forceinline__ void ray_pface(f3 o, f3 d, f3 c, float r, f3 cj, float rj, float &t, float &dp) {
+    pface_t(d, pface_plane(o, c, r, cj, rj), t, dp);
 }
 
 // rendering_math.py:107-131.

@@ -455,12 +455,24 @@ __device__ __forceinline__ float wave_sum_last(float x) {
     return x;
 }
 
+// The planes of faces [e, e + min(n, 64)) of a row, one face per lane:
+// normal and off - o.n (pface_plane), which every ray of the tile shares.
+template <typename F>
+__device__ __forceinline__ void stage_planes_t(float4 *splane, const RasterParams &p, f3 o, f3 c, float r, int e,
+                                               int n) {
+    if ((int)threadIdx.x < n) {
+        f3 dp3;
+        float rj;
+        RFace<F>::get(reinterpret_cast<const typename RFace<F>::T *>(p.faces)[e + threadIdx.x], dp3, rj);
+        splane[threadIdx.x] = pface_plane(o, c, r, c + dp3, rj);
+    }
+}
+
 // ------------------------------------------------------------- forward --
 template <typename F>
 __global__ __launch_bounds__(64) void raster_fwd_kernel(RasterParams p) {
-    using FT = typename RFace<F>::T;
     __shared__ PrimChunk pc;
-    __shared__ FT sface[64];
+    __shared__ float4 splane[64];
     __shared__ float tex[7 * 16];
     const TileCtx tc = tile_ctx(p);
     if (tc.tile >= p.ntiles) return;
@@ -499,16 +511,13 @@ __global__ __launch_bounds__(64) void raster_fwd_kernel(RasterParams p) {
         const int e0 = pc.e0[sub], n_adj = pc.e1[sub] - e0;
         for (int fb = 0; fb < n_adj; fb += 64) {
             __syncthreads();
-            if (fb + threadIdx.x < n_adj) sface[threadIdx.x] = reinterpret_cast<const FT *>(p.faces)[e0 + fb + threadIdx.x];
+            stage_planes_t<F>(splane, p, o, c, r, e0 + fb, n_adj - fb);
             __syncthreads();
             const int m = n_adj - fb < 64 ? n_adj - fb : 64;
             if (sh.hit) {
                 for (int j = 0; j < m; ++j) {
-                    f3 dp3;
-                    float rj;
-                    RFace<F>::get(sface[j], dp3, rj);
                     float t, dp;
-                    ray_pface(o, d, c, r, c + dp3, rj, t, dp);
+                    pface_t(d, splane[j], t, dp);
                     tf = dp >= 0.0f ? fmin_(t, tf) : tf;
                     tn = dp < 0.0f ? fmax_(t, tn) : tn;
                 }
@@ -711,9 +720,8 @@ static_assert(kRedCols == 64, "one column per lane");
 // spilled) measured 46.3 against 45.7 ms.
 template <typename F, int KT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void raster_bwd_kernel(RasterParams p) {
-    using FT = typename RFace<F>::T;
     __shared__ PrimChunk pc;
-    __shared__ FT sface[64];
+    __shared__ float4 splane[64];
     __shared__ float tex[7 * 8];
     __shared__ float sadj[64 * 4];
     const TileCtx tc = tile_ctx(p);
@@ -753,16 +761,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
         const int e0 = pc.e0[sub], n_adj = pc.e1[sub] - e0;
         for (int fb = 0; fb < n_adj; fb += 64) {
             __syncthreads();
-            if (fb + threadIdx.x < n_adj) sface[threadIdx.x] = reinterpret_cast<const FT *>(p.faces)[e0 + fb + threadIdx.x];
+            stage_planes_t<F>(splane, p, o, c, r, e0 + fb, n_adj - fb);
             __syncthreads();
             const int m = n_adj - fb < 64 ? n_adj - fb : 64;
             if (sh.hit) {
                 for (int j = 0; j < m; ++j) {
-                    f3 dp3;
-                    float rj;
-                    RFace<F>::get(sface[j], dp3, rj);
                     float t, dp;
-                    ray_pface(o, d, c, r, c + dp3, rj, t, dp);
+                    pface_t(d, splane[j], t, dp);
                     tf_id = (dp >= 0.0f && t < tf) ? fb + j : tf_id;
                     tf = dp >= 0.0f ? fmin_(t, tf) : tf;
                     tn_id = (dp < 0.0f && t > tn) ? fb + j : tn_id;
