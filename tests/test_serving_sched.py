@@ -98,3 +98,21 @@ def test_grouped_backward_limits():
         active = [r for r in active if r._next_tile < r.ntiles]
     for r in a:
         assert seen[r.rid] == list(range(r.ntiles))
+
+
+def test_steady_host_turns_the_collector_off_and_back():
+    import gc
+
+    from powerfoam.serving import steady_host
+
+    gc.enable()
+    with steady_host():
+        assert not gc.isenabled()
+    assert gc.isenabled()
+    gc.disable()
+    try:
+        with steady_host():
+            assert not gc.isenabled()
+        assert not gc.isenabled()  # left as it was found
+    finally:
+        gc.enable()
