@@ -298,10 +298,16 @@ __device__ __forceinline__ RTex texel_fwd_raster(f3 o, f3 d, float t_near, f3 c,
     const float tq0 = out.dp0 >= 0.0f ? t_near : fmax_(t_near, out.t0);
     const f3 q0 = ray_at(o, tq0, d);
     const float r2 = r * r;
+    // exp(-kTemp |q - s|^2 / r^2) as the hardware base-2 exponential of one
+    // product per texel (the scale folded once per cell), where a correctly
+    // rounded division and a software exp took about thirty VALU
+    // instructions; weights agree to a few ulp and the rasterizer carries a
+    // tolerance, not bit-exactness (the walk keeps the exact form)
+    const float ke = (-kTemp * 1.44269504088896341f) / r2;
     float hs = 0.0f, ws = 0.0f;
     auto height_term = [&](int i) {
         const f3 dv = q0 - mk(tex[3 * i], tex[3 * i + 1], tex[3 * i + 2]);
-        const float w = pf_expf(-kTemp * (dot(dv, dv) / r2));
+        const float w = __builtin_amdgcn_exp2f(ke * dot(dv, dv));
         hs = __builtin_fmaf(w, tex[6 * K + i], hs);
         ws = ws + w;
         return w;
@@ -325,7 +331,7 @@ __device__ __forceinline__ RTex texel_fwd_raster(f3 o, f3 d, float t_near, f3 c,
     float ws2 = 0.0f;
     auto colour_term = [&](int i) {
         const f3 dv = q - mk(tex[3 * i], tex[3 * i + 1], tex[3 * i + 2]);
-        const float w = pf_expf(-kTemp * (dot(dv, dv) / r2));
+        const float w = __builtin_amdgcn_exp2f(ke * dot(dv, dv));
         acc = macc(acc, w, mk(tex[3 * K + 3 * i], tex[3 * K + 3 * i + 1], tex[3 * K + 3 * i + 2]));
         ws2 = ws2 + w;
         return w;
@@ -465,7 +471,36 @@ __device__ __forceinline__ void stage_planes_t(float4 *splane, const RasterParam
         float rj;
         RFace<F>::get(reinterpret_cast<const typename RFace<F>::T *>(p.faces)[e + threadIdx.x], dp3, rj);
         splane[threadIdx.x] = pface_plane(o, c, r, c + dp3, rj);
+    } else if ((int)threadIdx.x < ((n + 3) & ~3)) {
+        // neutral planes up to a multiple of four: a NaN crossing, which the
+        // strict compares of every fold pass over (its zero normal's dot
+        // product may be -0, so an infinite offset could read as -inf)
+        splane[threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, __builtin_nanf(""));
     }
+}
+
+// The face planes splane[0, m) in order, read four at a time so that four
+// LDS reads are in flight where one at a time each face waits out its own
+// (the row is padded with neutral planes to a multiple of four).
+template <typename Fn> __device__ __forceinline__ void for_planes(const float4 *splane, int m, Fn &&fn) {
+    for (int j = 0; j < m; j += 4) {
+        const float4 a0 = splane[j], a1 = splane[j + 1], a2 = splane[j + 2], a3 = splane[j + 3];
+        fn(j, a0);
+        fn(j + 1, a1);
+        fn(j + 2, a2);
+        fn(j + 3, a3);
+    }
+}
+
+// The raster's face test: the plane's offset times the hardware reciprocal
+// of the ray's dot product with its normal (v_rcp_f32, 1 ulp), two VALU
+// instructions where a correctly rounded division takes about ten.  The
+// rasterizer carries a tolerance, not bit-exactness; forward and backward
+// both use this form, so they agree on every segment.  A zero dot product
+// gives an infinite (or, over a zero offset, NaN) crossing, as division does.
+__device__ __forceinline__ void pface_t_rcp(f3 d, float4 pl, float &t, float &dp) {
+    dp = dot(d, mk(pl.x, pl.y, pl.z));
+    t = pl.w * __builtin_amdgcn_rcpf(dp);
 }
 
 // ------------------------------------------------------------- forward --
@@ -508,19 +543,20 @@ __global__ __launch_bounds__(64) void raster_fwd_kernel(RasterParams p) {
         SphereHit sh = ray_sphere(o, d, c, r);
         if (!__any(sh.hit)) continue;  // tile_sum(hit) == 0 (:1029-1032)
         float tn = sh.tn, tf = sh.tf;
-        const int e0 = pc.e0[sub], n_adj = pc.e1[sub] - e0;
+        const int e0 = __builtin_amdgcn_readfirstlane(pc.e0[sub]);
+        const int n_adj = __builtin_amdgcn_readfirstlane(pc.e1[sub]) - e0;
         for (int fb = 0; fb < n_adj; fb += 64) {
             __syncthreads();
             stage_planes_t<F>(splane, p, o, c, r, e0 + fb, n_adj - fb);
             __syncthreads();
             const int m = n_adj - fb < 64 ? n_adj - fb : 64;
             if (sh.hit) {
-                for (int j = 0; j < m; ++j) {
+                for_planes(splane, m, [&](int, float4 pl) {
                     float t, dp;
-                    pface_t(d, splane[j], t, dp);
+                    pface_t_rcp(d, pl, t, dp);
                     tf = dp >= 0.0f ? fmin_(t, tf) : tf;
                     tn = dp < 0.0f ? fmax_(t, tn) : tn;
-                }
+                });
             }
         }
         stage_texels(tex, p, prim);
@@ -602,8 +638,9 @@ __device__ __forceinline__ void texel_bwd_raster(f3 o, f3 d, float tn_in, f3 c, 
     f3 dq = mk(0, 0, 0);
     const float gdp_in = dot(g_rgb, fw.rgb);
     // The forward's weights (no second exp), and reciprocals in place of the
-    // per-texel divisions; the 1e-3 skip keeps its exact division, so the set
-    // of contributing texels is the forward's.
+    // per-texel divisions.  The skip of texels under 1e-3 of the weight sum
+    // (texture.py:118, :158 divide) compares w with 1e-3 * sum, which differs
+    // from the quotient's test only where the quotient rounds across 1e-3.
     const float k2 = (2.0f * kTemp) * (1.0f / r2);  // d w / d s = k2 w (q - s)
     const float inv_ws = 1.0f / fw.wsum;
 #pragma unroll
@@ -611,7 +648,7 @@ __device__ __forceinline__ void texel_bwd_raster(f3 o, f3 d, float tn_in, f3 c, 
         if (KT == 0 && i >= K) break;
         const f3 dv = q - mk(tex[3 * i], tex[3 * i + 1], tex[3 * i + 2]);
         const float w = fw.w1[i];
-        if (w / fw.wsum < 1e-3f) continue;
+        if (w < 1e-3f * fw.wsum) continue;
         const f3 dwds = (k2 * w) * dv;
         const f3 grgb = (w * inv_ws) * g_rgb;
         red[3 * K + 3 * i] += grgb.x;
@@ -646,7 +683,7 @@ __device__ __forceinline__ void texel_bwd_raster(f3 o, f3 d, float tn_in, f3 c, 
         if (KT == 0 && i >= K) break;
         const f3 dv = q0 - mk(tex[3 * i], tex[3 * i + 1], tex[3 * i + 2]);
         const float w = fw.w0[i];
-        if (w / fw.wsum0 < 1e-3f) continue;
+        if (w < 1e-3f * fw.wsum0) continue;
         const f3 dwds = (k2 * w) * dv;
         red[6 * K + i] += (w * dh) * inv_ws0;
         const float hdp = dh * tex[6 * K + i];
@@ -758,21 +795,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
         float tn = sh.tn, tf = sh.tf;
         int tn_id = tn < 1e-8f ? -2 : -1;
         int tf_id = tf < 1e-8f ? -2 : -1;
-        const int e0 = pc.e0[sub], n_adj = pc.e1[sub] - e0;
+        const int e0 = __builtin_amdgcn_readfirstlane(pc.e0[sub]);
+        const int n_adj = __builtin_amdgcn_readfirstlane(pc.e1[sub]) - e0;
         for (int fb = 0; fb < n_adj; fb += 64) {
             __syncthreads();
             stage_planes_t<F>(splane, p, o, c, r, e0 + fb, n_adj - fb);
             __syncthreads();
             const int m = n_adj - fb < 64 ? n_adj - fb : 64;
             if (sh.hit) {
-                for (int j = 0; j < m; ++j) {
+                for_planes(splane, m, [&](int j, float4 pl) {
                     float t, dp;
-                    pface_t(d, splane[j], t, dp);
+                    pface_t_rcp(d, pl, t, dp);
                     tf_id = (dp >= 0.0f && t < tf) ? fb + j : tf_id;
                     tf = dp >= 0.0f ? fmin_(t, tf) : tf;
                     tn_id = (dp < 0.0f && t > tn) ? fb + j : tn_id;
                     tn = dp < 0.0f ? fmax_(t, tn) : tn;
-                }
+                });
             }
         }
         const float tn0 = tn;
