@@ -14,7 +14,8 @@ gradient + walk backward (the recorded segments sorted by cell and
 differentiated in cell order; rays past the record's cap replayed from the
 checkpoint in the same launch) + SH backward, inputs resident in HBM.  The
 same step with the record read ray by ray, and with checkpoint-and-replay
-and no record, is timed beside it ("backward_forms").  Also reported: the
+and no record (ray by ray, or band by band through the grouped pipeline),
+is timed beside it ("backward_forms").  Also reported: the
 16-camera differentiation configuration (262,144 cells, 800x800), the tile
 rasterizer on the same frame, and the two serving traces.  Prints ONE JSON
 line.  Single GPU: --gpus above 1 exits with a message.
@@ -72,6 +73,8 @@ def main():
                          "with the adjacency beside it (f16)")
     ap.add_argument("--record-cap", type=int, default=128,
                     help="contributing segments per ray the forward records for the backward (0 = replay)")
+    ap.add_argument("--replay-band-tiles", type=int, default=0,
+                    help="tiles per band of the banded replay backward form (0: the library default)")
     ap.add_argument("--backward", choices=("grouped", "recorded", "replay"), default="grouped",
                     help="walk backward: the record's segments sorted by cell (default), the record ray by ray, "
                          "or checkpoint-and-replay without a record")
@@ -104,6 +107,7 @@ def main():
     import numpy as np
     import torch
 
+    from powerfoam import _lib
     from powerfoam import raytrace as rt
     from powerfoam import shading, workloads
     from powerfoam.adjacency import build_power_adjacency
@@ -161,9 +165,12 @@ def main():
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     times = {"fwd": [], "bwd": []}
 
+    replay_ws = []  # the banded replay's workspace, allocated on first use
+    band = args.replay_band_tiles or rt.REPLAY_BAND_TILES
+
     def step(record, kind=None):
         kind = kind or args.backward
-        cap = 0 if kind == "replay" else args.record_cap
+        cap = 0 if kind in ("replay", "replay_grouped") else args.record_cap
         shading.sh_forward(coeffs, 3, sites, cams_dev, out=texel_rgb)
         if record:
             ev[0].record()
@@ -176,9 +183,16 @@ def main():
         g_coeffs.zero_()
         if record:
             ev[2].record()
-        b = rt.launch_walk_backward(scene, cams_dev, W, H, f["color"], f["log_t"], gc, None, grads=grads,
-                                    record=f["record"], grouped=kind == "grouped",
-                                    workspace=workspace if kind == "grouped" else None)
+        if kind == "replay_grouped":
+            if not replay_ws:
+                n = _lib.load().pf_walk_replay_grouped_workspace_bytes(args.cells, 1, W, H, 128, band)
+                replay_ws.append(torch.empty(n, dtype=torch.uint8, device=dev))
+            b = rt.launch_walk_backward(scene, cams_dev, W, H, f["color"], f["log_t"], gc, None, grads=grads,
+                                        grouped=True, workspace=replay_ws[0], band_tiles=band)
+        else:
+            b = rt.launch_walk_backward(scene, cams_dev, W, H, f["color"], f["log_t"], gc, None, grads=grads,
+                                        record=f["record"], grouped=kind == "grouped",
+                                        workspace=workspace if kind == "grouped" else None)
         if record:
             ev[3].record()
         shading.sh_backward(coeffs, 3, sites, cams_dev, grads["texel_rgb"][None], grad_coeffs=g_coeffs)
@@ -205,9 +219,10 @@ def main():
     mrays = rays / dt / 1e6
 
     # the other backward forms on the same step: the record read ray by ray,
-    # and checkpoint-and-replay without a record (no segment stored)
+    # and checkpoint-and-replay without a record (no segment kept from the
+    # forward), ray by ray or through the grouped pipeline band by band
     forms = {}
-    for kind in ("grouped", "recorded", "replay"):
+    for kind in ("grouped", "recorded", "replay", "replay_grouped"):
         if kind != "grouped" and args.record_cap <= 0:
             continue
         for _ in range(2):

@@ -12,7 +12,9 @@ struct WalkParams {
     pf_scene s;
     const float *cams;  // (B, 12) pf_camera
     int B, W, H, tiles_h, tiles_per_cam;
-    int64_t total_tiles;
+    // a launch covers tiles [tile0, tile0 + total_tiles) of the cameras'
+    // tile order; record slots count from the launch's first tile
+    int64_t total_tiles, tile0;
     float thr, depth_q;
     int max_steps;
     pf_walk_out out;
@@ -31,6 +33,10 @@ struct WalkParams {
     const int *rec_header;  // optional stamp; [7] holds the backward's verdict
 };
 
+// The recording forward over a launch's tiles (walk.hip), for callers in
+// other files that replay a walk band by band.
+int walk_record_launch(const WalkParams &p, hipStream_t st);
+
 struct TileLane {
     int b, pix_i, pix_j;
     bool valid;
@@ -45,8 +51,9 @@ __device__ __forceinline__ TileLane tile_lane(const WalkParams &p, int waves_per
     const int64_t t = (int64_t)block * waves_per_block + wave;
     tl.valid = t < p.total_tiles;
     const int64_t tt = tl.valid ? t : 0;
-    tl.b = (int)(tt / p.tiles_per_cam);
-    const int tile = (int)(tt % p.tiles_per_cam);
+    const int64_t tg = p.tile0 + tt;
+    tl.b = (int)(tg / p.tiles_per_cam);
+    const int tile = (int)(tg % p.tiles_per_cam);
     const int tile_i = tile % p.tiles_h;
     const int tile_j = tile / p.tiles_h;
     tl.pix_i = tile_i * 8 + (lane & 7);

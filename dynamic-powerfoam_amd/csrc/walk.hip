@@ -348,6 +348,8 @@ static int dispatch_walk(const WalkParams &p, WalkMode m, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? PF_OK : PF_ERR_HIP;
 }
 
+int pf::walk_record_launch(const WalkParams &p, hipStream_t st) { return dispatch_walk(p, kFwdRecord, st); }
+
 static int check_scene(const pf_scene *s) {
     if (!s || !s->spheres || !s->nsigmas || !s->texel_sites || !s->texel_rgb || !s->texel_height || !s->adjacency_offsets)
         return PF_ERR_ARG;

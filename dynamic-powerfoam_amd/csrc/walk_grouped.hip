@@ -66,7 +66,7 @@ struct SlotRay {
 };
 __device__ __forceinline__ SlotRay slot_ray(const WalkParams &p, int64_t slot) {
     SlotRay r;
-    const int64_t t = slot >> 6;
+    const int64_t t = p.tile0 + (slot >> 6);
     const int lane = (int)(slot & 63);
     r.b = (int)(t / p.tiles_per_cam);
     const int tile = (int)(t % p.tiles_per_cam);
@@ -288,11 +288,10 @@ struct GroupLayout {
     int64_t nb, nblk;
 };
 
-static bool group_layout(int64_t num_cells, int32_t num_cams, int32_t width, int32_t height, int32_t cap,
-                         bool per_camera, GroupLayout &L) {
-    if (num_cells <= 0 || num_cells >= 0x7FFFFFFF || num_cams <= 0 || width <= 0 || height <= 0 || cap <= 0)
-        return false;
-    const int64_t S = (int64_t)num_cams * ((height + 7) / 8) * ((width + 7) / 8) * 64;
+// S record slots (64 per tile), buckets per cell or per (camera, cell).
+static bool group_layout_slots(int64_t num_cells, int64_t S, int32_t num_cams, int32_t cap, bool per_camera,
+                               GroupLayout &L) {
+    if (num_cells <= 0 || num_cells >= 0x7FFFFFFF || num_cams <= 0 || S <= 0 || cap <= 0) return false;
     const int64_t nb = per_camera ? num_cells * num_cams : num_cells;
     // entry ids and bucket offsets are int32
     if (S * cap >= (int64_t(1) << 31) || nb >= 0x7FFFFFFF) return false;
@@ -313,6 +312,49 @@ static bool group_layout(int64_t num_cells, int32_t num_cams, int32_t width, int
     o = align256(o + (int64_t)sizeof(GroupEntry) * S * cap);
     L.total = o;
     return true;
+}
+
+static bool group_layout(int64_t num_cells, int32_t num_cams, int32_t width, int32_t height, int32_t cap,
+                         bool per_camera, GroupLayout &L) {
+    if (num_cams <= 0 || width <= 0 || height <= 0) return false;
+    const int64_t S = (int64_t)num_cams * ((height + 7) / 8) * ((width + 7) / 8) * 64;
+    return group_layout_slots(num_cells, S, num_cams, cap, per_camera, L);
+}
+
+// The grouped pipeline over q's record slots [0, q.S): bucket counts, scan,
+// scatter of the sorted entries, gradient pass; its workspace is carved
+// from ws by L.
+static int run_grouped(GroupParams &q, const GroupLayout &L, char *ws, hipStream_t st) {
+    q.ws.bucket_off = reinterpret_cast<int *>(ws + L.off_bucket);
+    q.ws.cursor = reinterpret_cast<int *>(ws + L.off_cursor);
+    q.ws.blk = reinterpret_cast<int *>(ws + L.off_blk);
+    q.ws.over_count = reinterpret_cast<int *>(ws + L.off_over_count);
+    q.ws.over_list = reinterpret_cast<int *>(ws + L.off_over_list);
+    q.ws.ent = reinterpret_cast<GroupEntry *>(ws + L.off_ent);
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    q.grouped_blocks = cus * kGroupBlocksPerCU;
+    const unsigned slot_blocks = (unsigned)((q.S + 255) / 256);
+    hipLaunchKernelGGL(bucket_zero_kernel, dim3(1024), dim3(256), 0, st, bucket_scan_of(q), q.ws.over_count, 1);
+    hipLaunchKernelGGL(grp_bucket_kernel<false>, dim3(slot_blocks), dim3(256), 0, st, q);
+    bucket_scan(bucket_scan_of(q), (unsigned)q.nblk, st);
+    hipLaunchKernelGGL(grp_bucket_kernel<true>, dim3(slot_blocks), dim3(256), 0, st, q);
+    const dim3 grid((unsigned)(kReplayBlocks + q.grouped_blocks));
+    const pf_scene &sc = q.w.s;
+    if (sc.attr_dtype == PF_F32) {
+        if (sc.face_dtype == PF_F16_PACKED)
+            hipLaunchKernelGGL((grp_grad_kernel<float, PackedF16>), grid, dim3(256), 0, st, q);
+        else if (sc.face_dtype == PF_F16)
+            hipLaunchKernelGGL((grp_grad_kernel<float, _Float16>), grid, dim3(256), 0, st, q);
+        else hipLaunchKernelGGL((grp_grad_kernel<float, float>), grid, dim3(256), 0, st, q);
+    } else {
+        if (sc.face_dtype == PF_F16_PACKED)
+            hipLaunchKernelGGL((grp_grad_kernel<_Float16, PackedF16>), grid, dim3(256), 0, st, q);
+        else if (sc.face_dtype == PF_F16)
+            hipLaunchKernelGGL((grp_grad_kernel<_Float16, _Float16>), grid, dim3(256), 0, st, q);
+        else hipLaunchKernelGGL((grp_grad_kernel<_Float16, float>), grid, dim3(256), 0, st, q);
+    }
+    return hipGetLastError() == hipSuccess ? PF_OK : PF_ERR_HIP;
 }
 
 extern "C" int64_t pf_walk_grouped_workspace_bytes(int64_t num_cells, int32_t num_cams, int32_t width,
@@ -450,38 +492,129 @@ static int grouped_backward(const pf_scene *scene, const pf_camera *cams, int32_
     q.S = p.total_tiles * 64;
     q.nb = (int)L.nb;
     q.nblk = (int)L.nblk;
-    char *ws = reinterpret_cast<char *>(workspace);
-    q.ws.bucket_off = reinterpret_cast<int *>(ws + L.off_bucket);
-    q.ws.cursor = reinterpret_cast<int *>(ws + L.off_cursor);
-    q.ws.blk = reinterpret_cast<int *>(ws + L.off_blk);
-    q.ws.over_count = reinterpret_cast<int *>(ws + L.off_over_count);
-    q.ws.over_list = reinterpret_cast<int *>(ws + L.off_over_list);
-    q.ws.ent = reinterpret_cast<GroupEntry *>(ws + L.off_ent);
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    q.grouped_blocks = cus * kGroupBlocksPerCU;
-
     hipStream_t st = (hipStream_t)stream;
-    int rc = pf_walk_check_stamp(record, scene, cams, num_cams, p.total_tiles, status, st);
+    const int rc = pf_walk_check_stamp(record, scene, cams, num_cams, p.total_tiles, status, st);
     if (rc) return rc;
-    const unsigned slot_blocks = (unsigned)((q.S + 255) / 256);
-    hipLaunchKernelGGL(bucket_zero_kernel, dim3(1024), dim3(256), 0, st, bucket_scan_of(q), q.ws.over_count, 1);
-    hipLaunchKernelGGL(grp_bucket_kernel<false>, dim3(slot_blocks), dim3(256), 0, st, q);
-    bucket_scan(bucket_scan_of(q), (unsigned)q.nblk, st);
-    hipLaunchKernelGGL(grp_bucket_kernel<true>, dim3(slot_blocks), dim3(256), 0, st, q);
-    const dim3 grid((unsigned)(kReplayBlocks + q.grouped_blocks));
-    if (scene->attr_dtype == PF_F32) {
-        if (scene->face_dtype == PF_F16_PACKED)
-            hipLaunchKernelGGL((grp_grad_kernel<float, PackedF16>), grid, dim3(256), 0, st, q);
-        else if (scene->face_dtype == PF_F16)
-            hipLaunchKernelGGL((grp_grad_kernel<float, _Float16>), grid, dim3(256), 0, st, q);
-        else hipLaunchKernelGGL((grp_grad_kernel<float, float>), grid, dim3(256), 0, st, q);
-    } else {
-        if (scene->face_dtype == PF_F16_PACKED)
-            hipLaunchKernelGGL((grp_grad_kernel<_Float16, PackedF16>), grid, dim3(256), 0, st, q);
-        else if (scene->face_dtype == PF_F16)
-            hipLaunchKernelGGL((grp_grad_kernel<_Float16, _Float16>), grid, dim3(256), 0, st, q);
-        else hipLaunchKernelGGL((grp_grad_kernel<_Float16, float>), grid, dim3(256), 0, st, q);
+    return run_grouped(q, L, reinterpret_cast<char *>(workspace), st);
+}
+
+// ----------------------------------- checkpoint and replay, grouped --
+// The backward with no record from the forward: its outputs (colour and
+// log T per ray) are the checkpoint, and the walk is replayed band by band
+// (band_tiles tiles of 8x8 rays, in the cameras' tile order) by the
+// recording forward into a band-sized record, whose segments the grouped
+// pipeline above sorts by cell and differentiates.  Gradients accumulate
+// over the bands (every gradient store is an atomic add).  The workspace
+// holds the replay's colours, one band's record and its sorted entries, so
+// its size follows the band, not the frame.
+struct ReplayLayout {
+    int64_t off_color, off_counts, off_segs, off_part, off_group, total;
+    GroupLayout g;
+};
+
+static bool replay_layout(int64_t num_cells, int32_t num_cams, int32_t width, int32_t height, int32_t cap,
+                          int64_t band, bool per_camera, ReplayLayout &R) {
+    if (num_cams <= 0 || width <= 0 || height <= 0 || cap <= 0 || cap % kRecChunk || band <= 0) return false;
+    const int64_t S = band * 64;
+    if (!group_layout_slots(num_cells, S, num_cams, cap, per_camera, R.g)) return false;
+    int64_t o = 0;
+    R.off_color = o;
+    o = align256(o + 12 * (int64_t)num_cams * width * height);
+    R.off_counts = o;
+    o = align256(o + 4 * S);
+    R.off_segs = o;
+    o = align256(o + 16 * S * cap);
+    R.off_part = o;
+    o = align256(o + 16 * S * cap);
+    R.off_group = o;
+    R.total = align256(o + R.g.total);
+    return true;
+}
+
+static int64_t replay_band(int32_t num_cams, int32_t width, int32_t height, int32_t band_tiles) {
+    const int64_t total = (int64_t)num_cams * ((height + 7) / 8) * ((width + 7) / 8);
+    return band_tiles < total ? band_tiles : total;
+}
+
+extern "C" int64_t pf_walk_replay_grouped_workspace_bytes(int64_t num_cells, int32_t num_cams, int32_t width,
+                                                          int32_t height, int32_t cap, int32_t band_tiles) {
+    ReplayLayout R;
+    if (num_cams <= 0 || width <= 0 || height <= 0 || band_tiles <= 0) return 0;
+    // sized for per-camera buckets, so one workspace serves either layout
+    return replay_layout(num_cells, num_cams, width, height, cap, replay_band(num_cams, width, height, band_tiles),
+                         num_cams > 1, R)
+               ? R.total
+               : 0;
+}
+
+extern "C" int pf_walk_backward_replay_grouped(const pf_scene *scene, const pf_camera *cams, int32_t num_cams,
+                                               int32_t width, int32_t height, float transmittance_threshold,
+                                               int32_t max_steps, const float *color, const float *log_t,
+                                               const float *grad_color, const float *grad_log_t,
+                                               const pf_grads *grads, int32_t flags, int32_t record_cap,
+                                               int32_t band_tiles, void *workspace, int64_t workspace_bytes,
+                                               int32_t *status, void *stream) {
+    if (!scene || !cams || num_cams <= 0 || width <= 0 || height <= 0 || !color || !grads || max_steps <= 0 ||
+        band_tiles <= 0)
+        return PF_ERR_ARG;
+    if (!scene->spheres || !scene->nsigmas || !scene->texel_sites || !scene->texel_rgb || !scene->texel_height ||
+        !scene->adjacency_offsets || scene->num_cells <= 0 || scene->num_cells >= 0x7FFFFFFF)
+        return PF_ERR_ARG;
+    if (scene->num_edges > 0 && (!scene->adjacency || !scene->faces)) return PF_ERR_ARG;
+    if (scene->num_texel_sites != 8) return PF_ERR_ARG;
+    if (!grads->spheres || !grads->nsigmas || !grads->texel_sites || !grads->texel_rgb || !grads->texel_height)
+        return PF_ERR_ARG;
+    if ((int64_t)num_cams * ((height + 7) / 8) * ((width + 7) / 8) > (int64_t(1) << 25)) return PF_ERR_ARG;
+    if (scene->attr_dtype != PF_F32 && scene->attr_dtype != PF_F16) return PF_ERR_DTYPE;
+    if (scene->face_dtype != PF_F32 && scene->face_dtype != PF_F16 && scene->face_dtype != PF_F16_PACKED)
+        return PF_ERR_DTYPE;
+    const bool per_camera = grads->texel_rgb_cam_stride != 0 && num_cams > 1;
+    const int64_t band = replay_band(num_cams, width, height, band_tiles);
+    ReplayLayout R;
+    if (!replay_layout(scene->num_cells, num_cams, width, height, record_cap, band, per_camera, R)) return PF_ERR_ARG;
+    if (!workspace || workspace_bytes < R.total) return PF_ERR_OVERFLOW;
+    char *ws = reinterpret_cast<char *>(workspace);
+
+    GroupParams q{};
+    WalkParams &p = q.w;
+    p.s = *scene;
+    p.cams = reinterpret_cast<const float *>(cams);
+    p.B = num_cams;
+    p.W = width;
+    p.H = height;
+    p.tiles_h = (height + 7) / 8;
+    p.tiles_per_cam = p.tiles_h * ((width + 7) / 8);
+    p.thr = transmittance_threshold;
+    p.max_steps = max_steps;
+    p.status = status;
+    p.rec_segs = reinterpret_cast<int4 *>(ws + R.off_segs);
+    p.rec_counts = reinterpret_cast<int *>(ws + R.off_counts);
+    p.rec_cap = record_cap;
+    p.rec_part = reinterpret_cast<float4 *>(ws + R.off_part);
+    WalkParams f = p;  // the replay: the recording forward, colours to the workspace
+    f.out.color = reinterpret_cast<float *>(ws + R.off_color);
+    f.out.status = status;
+    p.ck_color = color;
+    p.ck_log_t = log_t;
+    p.g_color = grad_color;
+    p.g_log_t = grad_log_t;
+    p.grads = *grads;
+    p.exact_texel = (flags & PF_BWD_EXACT_TEXEL) != 0;
+    q.part = p.rec_part;
+    q.header = nullptr;
+    q.nb = (int)R.g.nb;
+    q.nblk = (int)R.g.nblk;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t total = (int64_t)p.tiles_per_cam * num_cams;
+    for (int64_t t0 = 0; t0 < total; t0 += band) {
+        const int64_t n = total - t0 < band ? total - t0 : band;
+        f.tile0 = p.tile0 = t0;
+        f.total_tiles = p.total_tiles = n;
+        q.S = n * 64;
+        int rc = walk_record_launch(f, st);
+        if (rc) return rc;
+        rc = run_grouped(q, R.g, ws + R.off_group, st);
+        if (rc) return rc;
     }
-    return hipGetLastError() == hipSuccess ? PF_OK : PF_ERR_HIP;
+    return PF_OK;
 }

@@ -166,6 +166,13 @@ _SIGS = {
          c_void_p],
     ),
     "pf_walk_grouped_scratch_bytes": (c_int64, []),
+    "pf_walk_backward_replay_grouped": (
+        c_int32,
+        [ctypes.POINTER(PfScene), c_void_p, c_int32, c_int32, c_int32, c_float, c_int32, c_void_p, c_void_p,
+         c_void_p, c_void_p, ctypes.POINTER(PfGrads), c_int32, c_int32, c_int32, c_void_p, c_int64, c_void_p,
+         c_void_p],
+    ),
+    "pf_walk_replay_grouped_workspace_bytes": (c_int64, [c_int64, c_int32, c_int32, c_int32, c_int32, c_int32]),
     "pf_walk_faces": (
         c_int32,
         [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p, c_int32, c_void_p],

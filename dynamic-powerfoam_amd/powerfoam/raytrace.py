@@ -278,14 +278,26 @@ def launch_walk_forward(scene, cams_dev, width, height, transmittance_threshold=
                 alg_bytes=alg_bytes, record=record, ray_status=ray_status)
 
 
+# Tiles per band of the banded replay backward: 8 192 tiles of 8x8 rays
+# hold 524 288 record slots, about 6.4 GB of workspace at cap 128.  On the
+# 1M-cell 1080p frame (32 400 tiles) its backward took 21.9 ms, against
+# 20.4 ms in one band and 30.3 ms in bands of 4 096 tiles, whose recording
+# forward launches no longer fill the chip.
+REPLAY_BAND_TILES = 8192
+
+
 def launch_walk_backward(scene, cams_dev, width, height, color, log_t, grad_color, grad_log_t,
                          transmittance_threshold=1e-3, max_steps=DEFAULT_MAX_STEPS, grads=None, exact_texel=False,
-                         texel_rgb_cam_stride=0, want_bytes=False, record=None, grouped=None, workspace=None):
+                         texel_rgb_cam_stride=0, want_bytes=False, record=None, grouped=None, workspace=None,
+                         band_tiles=REPLAY_BAND_TILES, replay_cap=128):
     """Gradients of the walk.  record: the forward's record (its "record"
     output), read instead of replaying every ray; None replays.  grouped:
     differentiate the record's segments sorted by cell
     (pf_walk_backward_grouped); None: whenever the record has partials and
-    K = 8.  workspace: its device workspace (allocated if None)."""
+    K = 8.  grouped=True with no record: checkpoint and replay through the
+    grouped pipeline (pf_walk_backward_replay_grouped), band_tiles 8x8
+    tiles at a time, each band's segments (cap replay_cap) held in the
+    workspace.  workspace: the device workspace (allocated if None)."""
     lib = _lib.load()
     n, K = scene.num_cells, scene.K
     dev = cams_dev.device
@@ -345,6 +357,24 @@ def launch_walk_backward(scene, cams_dev, width, height, color, log_t, grad_colo
                                           workspace.numel(), _lib.ptr(status), _lib.ptr(alg_bytes),
                                           _lib.stream_ptr())
         _lib.check(rc, "pf_walk_backward_grouped")
+    elif record is None and grouped:
+        if K != 8:
+            raise ValueError("the grouped backward needs K = 8 texels")
+        if want_bytes:
+            raise ValueError("the banded replay does not count algorithmic bytes")
+        nbytes = lib.pf_walk_replay_grouped_workspace_bytes(int(n), int(B), int(width), int(height), int(replay_cap),
+                                                            int(band_tiles))
+        if nbytes <= 0:
+            raise ValueError("sizes out of range for the banded replay backward")
+        if workspace is None:
+            workspace = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        rc = lib.pf_walk_backward_replay_grouped(ctypes.byref(scene.c), _lib.ptr(cams_dev), B, width, height,
+                                                 float(transmittance_threshold), int(max_steps), _lib.ptr(color),
+                                                 _lib.ptr(log_t), _lib.ptr(grad_color), _lib.ptr(grad_log_t),
+                                                 ctypes.byref(g), flags, int(replay_cap), int(band_tiles),
+                                                 _lib.ptr(workspace), workspace.numel(), _lib.ptr(status),
+                                                 _lib.stream_ptr())
+        _lib.check(rc, "pf_walk_backward_replay_grouped")
     elif record is not None:
         if record["counts"].numel() != record_slots(cams_dev.shape[0], width, height):
             raise ValueError("the record belongs to a walk of another size")

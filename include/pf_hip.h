@@ -282,6 +282,39 @@ int pf_walk_backward_grouped_noalloc(const pf_scene *scene,
  * pf_walk_backward_grouped_noalloc.                                        */
 int64_t pf_walk_grouped_scratch_bytes(void);
 
+/* Checkpoint-and-replay backward through the cell-grouped pipeline.  No
+ * reference counterpart, as pf_walk_backward: RayTracer defines only a
+ * forward (raytrace.py:115-241).  No record from the forward: color /
+ * log_t are the checkpoint, and the walk is replayed
+ * band_tiles tiles (8x8 rays) at a time, in the cameras' tile order, by
+ * the recording forward (cap record_cap, a multiple of 8) into a
+ * band-sized record that the grouped pipeline sorts by cell and
+ * differentiates.  Gradients are added into grads (zero them first).
+ * workspace: device memory of at least
+ * pf_walk_replay_grouped_workspace_bytes(...) bytes for the same sizes;
+ * it holds one band, so its size follows band_tiles.  K = 8 texels.
+ * Gradients equal pf_walk_backward's up to the order of float additions. */
+int pf_walk_backward_replay_grouped(const pf_scene *scene,
+                                    const pf_camera *cams, int32_t num_cams,
+                                    int32_t width, int32_t height,
+                                    float transmittance_threshold,
+                                    int32_t max_steps, const float *color,
+                                    const float *log_t,
+                                    const float *grad_color,
+                                    const float *grad_log_t,
+                                    const pf_grads *grads, int32_t flags,
+                                    int32_t record_cap, int32_t band_tiles,
+                                    void *workspace, int64_t workspace_bytes,
+                                    int32_t *status, void *stream);
+
+/* Workspace bytes of pf_walk_backward_replay_grouped (0: sizes out of
+ * range).                                                                  */
+int64_t pf_walk_replay_grouped_workspace_bytes(int64_t num_cells,
+                                               int32_t num_cams,
+                                               int32_t width, int32_t height,
+                                               int32_t cap,
+                                               int32_t band_tiles);
+
 /* Walk face precompute (benchmark.py:295-302): for every CSR edge i->j,
  * faces[e] = half(p_j - p_i, pm_j - pm_i) with pm = 0.5(|p|^2 - r^2).
  * face_dtype PF_F16 (reference), PF_F32 or PF_F16_PACKED.                  */
