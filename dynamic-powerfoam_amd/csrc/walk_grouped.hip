@@ -564,7 +564,8 @@ extern "C" int pf_walk_backward_replay_grouped(const pf_scene *scene, const pf_c
     if (scene->num_texel_sites != 8) return PF_ERR_ARG;
     if (!grads->spheres || !grads->nsigmas || !grads->texel_sites || !grads->texel_rgb || !grads->texel_height)
         return PF_ERR_ARG;
-    if ((int64_t)num_cams * ((height + 7) / 8) * ((width + 7) / 8) > (int64_t(1) << 25)) return PF_ERR_ARG;
+    // the walk launches' limit: fewer than 2^25 tiles per call
+    if ((int64_t)num_cams * ((height + 7) / 8) * ((width + 7) / 8) >= (int64_t(1) << 25)) return PF_ERR_ARG;
     if (scene->attr_dtype != PF_F32 && scene->attr_dtype != PF_F16) return PF_ERR_DTYPE;
     if (scene->face_dtype != PF_F32 && scene->face_dtype != PF_F16 && scene->face_dtype != PF_F16_PACKED)
         return PF_ERR_DTYPE;
