@@ -18,8 +18,7 @@ namespace pf {
 // (pf_walk_deferred.h); other K the per-lane walk.  Same results.  Five
 // waves per SIMD (at most 96 VGPRs; the recording walk needs 101 unbounded
 // and fits without spilling).
-template <typename A, typename F, int KT, bool REC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void walk_fwd_kernel(WalkParams p) {
+template <typename A, typename F, int KT, bool REC> __device__ __forceinline__ void walk_fwd_tile(const WalkParams &p) {
     __shared__ WaveShade shade[KT == 8 ? 4 : 1];
     const TileLane tl = tile_lane(p, 4, xcd_block(blockIdx.x, gridDim.x));
     if (!tl.valid) return;
@@ -49,6 +48,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     if (p.out.alg_bytes)
         p.out.alg_bytes[tl.ray] = r.bytes + 12 + (p.out.log_t ? 4 : 0) + (p.out.depth ? 4 : 0) + (REC ? 4 : 0);
     if (r.status && p.out.status) atomicOr(p.out.status, r.status);
+}
+
+template <typename A, typename F, int KT, bool REC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void walk_fwd_kernel(WalkParams p) {
+    walk_fwd_tile<A, F, KT, REC>(p);
+}
+
+// The same recording forward over one band of a banded replay backward, a
+// kernel of its own name so that profiles keep its band-sized launches
+// apart from the frame's.
+template <typename A, typename F, int KT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void walk_band_fwd_kernel(WalkParams p) {
+    walk_fwd_tile<A, F, KT, true>(p);
 }
 
 // Slots of the per-wave gradient cache (K = 8: 65-float rows, 9.9 KB of
@@ -316,7 +328,7 @@ int pf_walk_check_stamp(const pf_walk_record *record, const pf_scene *scene, con
     return record_stamp(record, scene, cams, num_cams, total_tiles, 1, status, st);
 }
 
-enum WalkMode { kFwd, kFwdRecord, kBwd, kBwdRecord };
+enum WalkMode { kFwd, kFwdRecord, kFwdBand, kBwd, kBwdRecord };
 
 template <typename A, typename F, int KT> static void launch_walk_k(const WalkParams &p, WalkMode m, hipStream_t st) {
     const int64_t blocks = (p.total_tiles + 3) / 4;
@@ -324,6 +336,7 @@ template <typename A, typename F, int KT> static void launch_walk_k(const WalkPa
     switch (m) {
     case kFwd: hipLaunchKernelGGL((walk_fwd_kernel<A, F, KT, false>), grid, block, 0, st, p); break;
     case kFwdRecord: hipLaunchKernelGGL((walk_fwd_kernel<A, F, KT, true>), grid, block, 0, st, p); break;
+    case kFwdBand: hipLaunchKernelGGL((walk_band_fwd_kernel<A, F, KT>), grid, block, 0, st, p); break;
     case kBwd: hipLaunchKernelGGL((walk_bwd_kernel<A, F, KT>), grid, block, 0, st, p); break;
     case kBwdRecord: hipLaunchKernelGGL((walk_bwd_rec_kernel<A, F, KT>), dim3(2 * (unsigned)blocks), block, 0, st, p);
         break;
@@ -348,7 +361,7 @@ static int dispatch_walk(const WalkParams &p, WalkMode m, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? PF_OK : PF_ERR_HIP;
 }
 
-int pf::walk_record_launch(const WalkParams &p, hipStream_t st) { return dispatch_walk(p, kFwdRecord, st); }
+int pf::walk_record_launch(const WalkParams &p, hipStream_t st) { return dispatch_walk(p, kFwdBand, st); }
 
 static int check_scene(const pf_scene *s) {
     if (!s || !s->spheres || !s->nsigmas || !s->texel_sites || !s->texel_rgb || !s->texel_height || !s->adjacency_offsets)
