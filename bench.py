@@ -73,6 +73,8 @@ def main():
                          "with the adjacency beside it (f16)")
     ap.add_argument("--record-cap", type=int, default=128,
                     help="contributing segments per ray the forward records for the backward (0 = replay)")
+    ap.add_argument("--forms", default="grouped,recorded,replay,replay_grouped",
+                    help="backward forms timed beside the step, comma-separated")
     ap.add_argument("--replay-band-tiles", type=int, default=0,
                     help="tiles per band of the banded replay backward form (0: the library default)")
     ap.add_argument("--backward", choices=("grouped", "recorded", "replay"), default="grouped",
@@ -222,7 +224,11 @@ def main():
     # and checkpoint-and-replay without a record (no segment kept from the
     # forward), ray by ray or through the grouped pipeline band by band
     forms = {}
-    for kind in ("grouped", "recorded", "replay", "replay_grouped"):
+    kinds = [k for k in args.forms.split(",") if k]
+    unknown = set(kinds) - {"grouped", "recorded", "replay", "replay_grouped"}
+    if unknown:
+        raise SystemExit(f"--forms: unknown backward forms {sorted(unknown)}")
+    for kind in kinds:
         if kind != "grouped" and args.record_cap <= 0:
             continue
         for _ in range(2):
